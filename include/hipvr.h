@@ -1,0 +1,172 @@
+/*
+ * hipvr.h -- C ABI of libhipvr.so, the MI355X (gfx950) direct volume
+ * renderer behind the Python VolumeRenderer.
+ *
+ * The reference renders with an OpenGL fragment shader driven through
+ * ModernGL; this ABI replaces that device-resource layer (the reference's
+ * ModernGLManager, pyvr/moderngl_renderer/manager.py:8-256) and the shader
+ * (pyvr/shaders/volume.frag.glsl).  Paths below are relative to the reference
+ * repository.
+ *
+ * Conventions
+ *   - Every function returns HVR_OK (0) or a negative HVR_E* code; the
+ *     message of the last failure on the calling thread is hvr_last_error().
+ *   - Host arrays are copied at upload (as the reference's tobytes() copies);
+ *     the caller keeps ownership of every host pointer.
+ *   - A context (hvr_ctx) is used by one host thread at a time, like a GL
+ *     context.  Different contexts may be used from different threads.
+ *   - Volumes belong to the context that created them and must be destroyed
+ *     before it.
+ */
+#ifndef HIPVR_H
+#define HIPVR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HVR_ABI_VERSION 1
+
+/* return codes */
+#define HVR_OK 0
+#define HVR_EINVAL -1   /* bad argument (null pointer, size, format)        */
+#define HVR_EDEVICE -2  /* HIP runtime error                                 */
+#define HVR_ENOMEM -3   /* device allocation failed                          */
+#define HVR_ESTATE -4   /* call not valid in the current state               */
+
+/* storage format of the scalar field in a volume */
+#define HVR_SCALAR_NONE 0
+#define HVR_SCALAR_F32 1   /* float32                                       */
+#define HVR_SCALAR_F16 2   /* IEEE half, round to nearest even              */
+#define HVR_SCALAR_U8 3    /* UNORM8: round(255 * clamp(v, 0, 1))           */
+
+/* storage format of the normal field in a volume */
+#define HVR_NORMAL_NONE 0
+#define HVR_NORMAL_F32X3 1 /* 3 x float32                                   */
+#define HVR_NORMAL_SN8X3 2 /* SNORM8: round(127 * clamp(n, -1, 1)) per axis */
+
+/* which field of a volume a sampler reads (see hvr_bind) */
+#define HVR_ROLE_SCALAR 0  /* the scalar field                               */
+#define HVR_ROLE_NORMAL 1  /* the normal field                               */
+
+typedef struct hvr_ctx hvr_ctx;
+typedef struct hvr_volume hvr_volume;
+
+/* Uniform block of one frame.  Replaces the shader uniforms of
+ * volume.frag.glsl:3-19 set by renderer.py:99-105, 136-141, 163-172,
+ * 303-316.  view/proj are the float32 matrices exactly as the reference
+ * uploads them (numpy row-major bytes of Camera.get_view_matrix() and
+ * Camera.get_projection_matrix(aspect), camera.py:292-369).  The library
+ * inverts them in double precision. */
+typedef struct {
+    float view[16];
+    float proj[16];
+    float cam_pos[3];
+    float bmin[3], bmax[3];        /* volume_min/max_bounds                  */
+    float step;                    /* step_size                              */
+    float ref_step;                /* reference_step_size                    */
+    int32_t max_steps;
+    float ambient, diffuse;        /* ambient_light, diffuse_light           */
+    float light_pos[3], light_target[3];
+    float et_threshold;            /* early termination alpha; the shader's  */
+                                   /* constant is 0.99 (volume.frag.glsl:87) */
+    int32_t camera_set;            /* 0: matrices never set; the shader then */
+                                   /* sees zero matrices and every pixel is 0 */
+    uint8_t clear_rgba[4];         /* framebuffer clear colour, 8-bit        */
+                                   /* (clear_framebuffer, manager.py:212-215) */
+    int32_t blend;                 /* 1: SRC_ALPHA, ONE_MINUS_SRC_ALPHA over */
+                                   /* the clear colour (setup_blending,      */
+                                   /* manager.py:217-220); 0: overwrite      */
+} hvr_params;
+
+/* ---- library / device ---- */
+
+/* Version of this ABI (HVR_ABI_VERSION). */
+int hvr_abi_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *hvr_last_error(void);
+
+/* Number of visible HIP devices. */
+int hvr_device_count(int *count);
+
+/* ---- context: replaces ModernGLManager.__init__ (manager.py:14-41:
+ * standalone context + RGBA8 framebuffer of width x height) ---- */
+int hvr_create(int device, int width, int height, hvr_ctx **out);
+int hvr_destroy(hvr_ctx *ctx);
+
+/* ---- volumes: replace create_volume_texture (manager.py:77-108) and
+ * create_normal_texture (manager.py:110-135) ----
+ * A volume is one 3-D texture of GL size (s0, s1, s2) = the numpy shape of
+ * the host array, holding a scalar field, a normal field, or both
+ * interleaved in one texel.  Formats are fixed at creation. */
+int hvr_volume_create(hvr_ctx *ctx, int s0, int s1, int s2, int scalar_fmt, int normal_fmt,
+                      hvr_volume **out);
+/* Upload the scalar field: `data` is s0*s1*s2 float32 values in numpy C
+ * order (the bytes the reference passes to texture3d). */
+int hvr_volume_upload_scalar(hvr_volume *vol, const float *data);
+/* Upload the normal field: s0*s1*s2*3 float32 values in numpy C order. */
+int hvr_volume_upload_normals(hvr_volume *vol, const float *data);
+/* Device bytes held by the volume's texels. */
+int hvr_volume_bytes(const hvr_volume *vol, uint64_t *bytes);
+int hvr_volume_destroy(hvr_volume *vol);
+
+/* ---- transfer function: replaces create_rgba_transfer_function_texture
+ * (manager.py:137-186) ----  `rgba` is n x 4 float32 (RGB from the colour
+ * LUT, A from the opacity LUT).  n == 0 unbinds the LUT (an unbound sampler
+ * reads (0, 0, 0, 1)). */
+int hvr_set_lut(hvr_ctx *ctx, const float *rgba, int n);
+
+/* ---- uniforms: replace set_uniform_* (manager.py:188-210) ---- */
+int hvr_set_params(hvr_ctx *ctx, const hvr_params *params);
+
+/* ---- sampler bindings: replace set_uniform_int("volume_texture" /
+ * "normal_volume", unit) (renderer.py:133, 146) ----
+ * The scalar sampler (volume.frag.glsl:93, `.r`) reads field `scalar_role`
+ * of `scalar_vol`: the scalar, or the x component of the normal field.  The
+ * normal sampler (:103, `.rgb`) reads field `normal_role` of `normal_vol`:
+ * the normal, or (scalar, 0, 0) -- what a 3-channel GL sampler returns from
+ * a 1-channel texture.  A NULL volume is an unbound sampler: the scalar
+ * reads 0 and the normal (0, 0, 0) (GL incomplete texture).  The volume
+ * must hold the requested field. */
+int hvr_bind(hvr_ctx *ctx, hvr_volume *scalar_vol, int scalar_role, hvr_volume *normal_vol,
+             int normal_role);
+
+/* ---- render: replaces clear_framebuffer + setup_blending + render_quad +
+ * read_pixels (manager.py:212-230; renderer.py:209-219) ----
+ * Writes width*height*4 bytes, RGBA8, bottom row first, into `out` and
+ * returns when they are there. */
+int hvr_render(hvr_ctx *ctx, uint8_t *out);
+
+/* Asynchronous form: enqueue the frame on the context's stream, leaving the
+ * image in the context's device framebuffer.  hvr_read copies it out (and
+ * waits). */
+int hvr_render_async(hvr_ctx *ctx);
+int hvr_read(hvr_ctx *ctx, uint8_t *out);
+int hvr_synchronize(hvr_ctx *ctx);
+
+/* Device pointer of the context's RGBA8 framebuffer (width*height*4 bytes). */
+int hvr_framebuffer(hvr_ctx *ctx, void **device_ptr);
+
+/* Samples composited per pixel for the current state (a separate counting
+ * pass; not part of hvr_render).  `counts` is width*height uint32, bottom
+ * row first. */
+int hvr_sample_counts(hvr_ctx *ctx, uint32_t *counts);
+
+/* Milliseconds of the last frame's march kernel and ray-setup kernel,
+ * measured with HIP events on the context's stream. */
+int hvr_last_kernel_ms(hvr_ctx *ctx, float *march_ms, float *setup_ms);
+
+/* Tuning knobs (no counterpart in the reference): log2 of the brick edge
+ * used by volumes created afterwards (0 = linear layout, 2 = 4^3 bricks,
+ * 3 = 8^3 bricks). */
+int hvr_set_brick_log2(hvr_ctx *ctx, int log2_edge);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HIPVR_H */

@@ -1,0 +1,585 @@
+// hipvr.hip -- libhipvr.so: C ABI (include/hipvr.h) over the gfx950 kernels
+// of hvr_device.h.  Host side: contexts (framebuffer, rays, LUT, stream,
+// events), volumes (device texels in brick layout), FP64 uniform
+// preparation and kernel dispatch.
+#include "hvr_device.h"
+#include "../../include/hipvr.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+using namespace hvr;
+
+// ---------------------------------------------------------------------------
+// errors
+static thread_local std::string g_err;
+
+static int fail(int code, const char *msg)
+{
+    g_err = msg;
+    return code;
+}
+static int fail_hip(hipError_t e, const char *where)
+{
+    g_err = std::string(where) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? HVR_ENOMEM : HVR_EDEVICE;
+}
+#define HIPCHK(expr)                                   \
+    do {                                               \
+        hipError_t e_ = (expr);                        \
+        if (e_ != hipSuccess) return fail_hip(e_, #expr); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// objects
+struct hvr_volume {
+    hvr_ctx *ctx;
+    int s0, s1, s2;
+    int fmt;         // TexFmt
+    int lb;          // log2 brick edge
+    int nbx, nby, nbz;
+    uint64_t ntexels;
+    void *d_texels;
+    bool scalar_loaded, normals_loaded;
+};
+
+struct hvr_ctx {
+    int device;
+    int W, H;
+    hipStream_t stream;
+    uchar4 *d_out;
+    RayRec *d_rays;
+    uint32_t *d_counts;
+    float4 *d_lut;
+    int lut_n;       // entries in d_lut (>= 1)
+    int lut_cap;
+    void *h_pinned;  // W*H*4 readback staging
+    float *d_stage;  // upload staging
+    size_t stage_bytes;
+    hipEvent_t ev[4];
+    bool events_recorded;
+    hvr_params params;
+    bool params_set;
+    hvr_volume *scalar_vol, *normal_vol;
+    int scalar_role, normal_role;
+    int brick_log2;
+};
+
+static const int kLutLdsMax = 4096;  // 64 KiB of LDS per block
+
+// ---------------------------------------------------------------------------
+// FP64 uniform preparation
+
+// Inverse of a 4x4 matrix (row-major, m[4*r + c]) by Gauss-Jordan elimination
+// with partial pivoting.  Returns false when singular.
+static bool inverse4(const double *m, double *inv)
+{
+    double a[4][8];
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 8; ++c) a[r][c] = c < 4 ? m[4 * r + c] : (c - 4 == r ? 1.0 : 0.0);
+    for (int col = 0; col < 4; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < 4; ++r)
+            if (fabs(a[r][col]) > fabs(a[piv][col])) piv = r;
+        if (a[piv][col] == 0.0) return false;
+        if (piv != col)
+            for (int c = 0; c < 8; ++c) {
+                double t = a[col][c];
+                a[col][c] = a[piv][c];
+                a[piv][c] = t;
+            }
+        const double d = a[col][col];
+        for (int c = 0; c < 8; ++c) a[col][c] /= d;
+        for (int r = 0; r < 4; ++r) {
+            if (r == col) continue;
+            const double f = a[r][col];
+            if (f != 0.0)
+                for (int c = 0; c < 8; ++c) a[r][c] -= f * a[col][c];
+        }
+    }
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) inv[4 * r + c] = a[r][c + 4];
+    return true;
+}
+
+// Uniforms -> kernel constants.  The matrices arrive as numpy row-major
+// float32 bytes that GLSL reads column-major, i.e. transposed
+// (manager.py:192); they are inverted in double and rounded once to float.
+static FrameConst frame_const(const hvr_ctx *c)
+{
+    FrameConst f;
+    memset(&f, 0, sizeof(f));
+    const hvr_params &p = c->params;
+    f.W = c->W;
+    f.H = c->H;
+    f.fW = (float)c->W;
+    f.fH = (float)c->H;
+    double pg[16], vg[16], pinv[16], vinv[16];
+    for (int r = 0; r < 4; ++r)
+        for (int k = 0; k < 4; ++k) {
+            pg[4 * r + k] = (double)p.proj[4 * k + r];
+            vg[4 * r + k] = (double)p.view[4 * k + r];
+        }
+    f.ok = c->params_set && p.camera_set && inverse4(pg, pinv) && inverse4(vg, vinv);
+    if (f.ok) {
+        for (int k = 0; k < 8; ++k) f.prow[k] = (float)pinv[k];
+        for (int r = 0; r < 3; ++r)
+            for (int k = 0; k < 3; ++k) f.rot[3 * r + k] = (float)vinv[4 * r + k];
+    }
+    for (int k = 0; k < 3; ++k) {
+        f.o[k] = p.cam_pos[k];
+        f.bmin[k] = p.bmin[k];
+        f.bmax[k] = p.bmax[k];
+        f.ext[k] = p.bmax[k] - p.bmin[k];  // float subtraction, as in the shader
+    }
+    f.step = p.step;
+    f.alpha_k = (float)((double)p.step / (double)p.ref_step);
+    f.ka = p.ambient;
+    f.kd = p.diffuse;
+    f.tau = p.et_threshold;
+    f.max_steps = p.max_steps;
+    double l[3], ll = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        l[k] = (double)p.light_target[k] - (double)p.light_pos[k];
+        ll += l[k] * l[k];
+    }
+    f.L_ok = ll > 0.0;
+    for (int k = 0; k < 3; ++k) f.L[k] = f.L_ok ? (float)(l[k] / sqrt(ll)) : 0.f;
+    for (int k = 0; k < 4; ++k) f.clear[k] = p.clear_rgba[k];
+    f.blend = p.blend != 0;
+    return f;
+}
+
+static TexDesc tex_desc(const hvr_volume *v, int role = HVR_ROLE_SCALAR)
+{
+    TexDesc t;
+    memset(&t, 0, sizeof(t));
+    if (!v) return t;
+    t.base = v->d_texels;
+    t.s0 = v->s0;
+    t.s1 = v->s1;
+    t.s2 = v->s2;
+    t.fs0 = (float)v->s0;
+    t.fs1 = (float)v->s1;
+    t.fs2 = (float)v->s2;
+    t.lb = v->lb;
+    const uint32_t b3 = 1u << (3 * v->lb);
+    t.bsy = (uint32_t)v->nbx * b3;
+    t.bsz = (uint32_t)v->nbx * (uint32_t)v->nby * b3;
+    t.fmt = v->fmt;
+    t.role = role;
+    return t;
+}
+
+// ---------------------------------------------------------------------------
+// kernel dispatch
+template <int SF, int NM, bool COUNT>
+static void launch_march_t(hvr_ctx *c, const FrameConst &fc, const TexDesc &ts, const TexDesc &tn,
+                           int ntx, int nblocks)
+{
+    const size_t lds = NM == NM_ANY ? 0 : (size_t)c->lut_n * sizeof(float4);
+    hipLaunchKernelGGL((k_march<SF, NM, COUNT>), dim3(nblocks), dim3(256), lds, c->stream, fc, ts, tn,
+                       (const float4 *)c->d_lut, c->lut_n, (const RayRec *)c->d_rays, c->d_out,
+                       c->d_counts, ntx, nblocks);
+}
+
+template <bool COUNT>
+static void launch_march(hvr_ctx *c, const FrameConst &fc)
+{
+    const int ntx = (c->W + 15) / 16, nty = (c->H + 15) / 16;
+    const int nblocks = ntx * nty;
+    const hvr_volume *S = c->scalar_vol, *N = c->normal_vol;
+    const TexDesc ts = tex_desc(S, c->scalar_role), tn = tex_desc(N, c->normal_role);
+    const TexDesc none = tex_desc(nullptr);
+    const bool small_lut = c->lut_n <= kLutLdsMax;
+    if (small_lut && !S && !N) {
+        launch_march_t<TF_NONE, NM_ZERO, COUNT>(c, fc, none, none, ntx, nblocks);
+        return;
+    }
+    // Fast paths: scalar field of one volume, normals from the same texels,
+    // (rho, 0, 0) of the same volume, or unbound.
+    if (small_lut && S && c->scalar_role == HVR_ROLE_SCALAR) {
+        int nm = -1;
+        if (!N) nm = NM_ZERO;
+        else if (N == S && c->normal_role == HVR_ROLE_SCALAR) nm = NM_ALIAS;
+        else if (N == S && c->normal_role == HVR_ROLE_NORMAL && fmt_has_scalar(S->fmt)) nm = NM_SAME;
+#define HVR_CASE(SF)                                                                        \
+    case SF:                                                                                \
+        if (nm == NM_ZERO) launch_march_t<SF, NM_ZERO, COUNT>(c, fc, ts, none, ntx, nblocks);   \
+        else if (nm == NM_ALIAS) launch_march_t<SF, NM_ALIAS, COUNT>(c, fc, ts, none, ntx, nblocks); \
+        else launch_march_t<SF, HVR_NM_SAME_OR_ALIAS(SF), COUNT>(c, fc, ts, none, ntx, nblocks);  \
+        return;
+#define HVR_NM_SAME_OR_ALIAS(SF) (fmt_has_normal(SF) ? NM_SAME : NM_ALIAS)
+        if (nm >= 0) {
+            switch (S->fmt) {
+                HVR_CASE(TF_S_F32)
+                HVR_CASE(TF_S_F16)
+                HVR_CASE(TF_S_U8)
+                HVR_CASE(TF_C_F32_F32)
+                HVR_CASE(TF_C_F16_F32)
+                HVR_CASE(TF_C_U8_F32)
+                HVR_CASE(TF_C_F32_SN8)
+                HVR_CASE(TF_C_F16_SN8)
+                HVR_CASE(TF_C_U8_SN8)
+            default: break;
+            }
+        }
+#undef HVR_CASE
+#undef HVR_NM_SAME_OR_ALIAS
+    }
+    launch_march_t<TF_NONE, NM_ANY, COUNT>(c, fc, ts, tn, ntx, nblocks);
+}
+
+static int enqueue_frame(hvr_ctx *c, bool count)
+{
+    const FrameConst fc = frame_const(c);
+    const int npix = c->W * c->H;
+    HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    hipLaunchKernelGGL(k_ray_setup, dim3((npix + 255) / 256), dim3(256), 0, c->stream, fc, c->d_rays);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev[1], c->stream));
+    if (count) launch_march<true>(c, fc);
+    else launch_march<false>(c, fc);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev[2], c->stream));
+    c->events_recorded = true;
+    return HVR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+extern "C" {
+
+int hvr_abi_version(void) { return HVR_ABI_VERSION; }
+
+const char *hvr_last_error(void) { return g_err.c_str(); }
+
+int hvr_device_count(int *count)
+{
+    if (!count) return fail(HVR_EINVAL, "count is null");
+    HIPCHK(hipGetDeviceCount(count));
+    return HVR_OK;
+}
+
+int hvr_create(int device, int width, int height, hvr_ctx **out)
+{
+    if (!out) return fail(HVR_EINVAL, "out is null");
+    *out = nullptr;
+    if (width <= 0 || height <= 0 || (long long)width * height > (1ll << 28))
+        return fail(HVR_EINVAL, "width and height must be positive (at most 2^28 pixels)");
+    HIPCHK(hipSetDevice(device));
+    hvr_ctx *c = new hvr_ctx();
+    memset(c, 0, sizeof(*c));
+    c->device = device;
+    c->W = width;
+    c->H = height;
+    c->brick_log2 = 2;
+    const size_t npix = (size_t)width * height;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&c->d_out, npix * sizeof(uchar4));
+    if (e == hipSuccess) e = hipMalloc(&c->d_rays, npix * sizeof(RayRec));
+    if (e == hipSuccess) e = hipHostMalloc(&c->h_pinned, npix * 4, hipHostMallocDefault);
+    for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
+    if (e == hipSuccess) {
+        // unbound LUT: a 1-texel (0, 0, 0, 1) texture (GL incomplete texture)
+        c->lut_cap = 1;
+        c->lut_n = 1;
+        e = hipMalloc(&c->d_lut, sizeof(float4));
+        const float4 unbound = make_float4(0.f, 0.f, 0.f, 1.f);
+        if (e == hipSuccess) e = hipMemcpy(c->d_lut, &unbound, sizeof(float4), hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) e = hipMemset(c->d_out, 0, npix * sizeof(uchar4));
+    if (e != hipSuccess) {
+        hvr_destroy(c);
+        return fail_hip(e, "hvr_create");
+    }
+    *out = c;
+    return HVR_OK;
+}
+
+int hvr_destroy(hvr_ctx *c)
+{
+    if (!c) return HVR_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(c->d_out);
+    (void)hipFree(c->d_rays);
+    (void)hipFree(c->d_counts);
+    (void)hipFree(c->d_lut);
+    (void)hipFree(c->d_stage);
+    if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+    for (int i = 0; i < 4; ++i)
+        if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return HVR_OK;
+}
+
+int hvr_set_brick_log2(hvr_ctx *c, int lb)
+{
+    if (!c) return fail(HVR_EINVAL, "ctx is null");
+    if (lb < 0 || lb > 4) return fail(HVR_EINVAL, "brick log2 must be in [0, 4]");
+    c->brick_log2 = lb;
+    return HVR_OK;
+}
+
+static int texel_format(int sf, int nf)
+{
+    if (nf == HVR_NORMAL_NONE) {
+        if (sf == HVR_SCALAR_F32) return TF_S_F32;
+        if (sf == HVR_SCALAR_F16) return TF_S_F16;
+        if (sf == HVR_SCALAR_U8) return TF_S_U8;
+        return -1;
+    }
+    if (nf == HVR_NORMAL_F32X3) {
+        if (sf == HVR_SCALAR_NONE) return TF_N_F32;
+        if (sf == HVR_SCALAR_F32) return TF_C_F32_F32;
+        if (sf == HVR_SCALAR_F16) return TF_C_F16_F32;
+        if (sf == HVR_SCALAR_U8) return TF_C_U8_F32;
+        return -1;
+    }
+    if (nf == HVR_NORMAL_SN8X3) {
+        if (sf == HVR_SCALAR_NONE) return TF_N_SN8;
+        if (sf == HVR_SCALAR_F32) return TF_C_F32_SN8;
+        if (sf == HVR_SCALAR_F16) return TF_C_F16_SN8;
+        if (sf == HVR_SCALAR_U8) return TF_C_U8_SN8;
+    }
+    return -1;
+}
+
+int hvr_volume_create(hvr_ctx *c, int s0, int s1, int s2, int sf, int nf, hvr_volume **out)
+{
+    if (!c || !out) return fail(HVR_EINVAL, "null argument");
+    *out = nullptr;
+    if (s0 <= 0 || s1 <= 0 || s2 <= 0) return fail(HVR_EINVAL, "volume dimensions must be positive");
+    const int fmt = texel_format(sf, nf);
+    if (fmt < 0) return fail(HVR_EINVAL, "unsupported scalar/normal format combination");
+    const int lb = c->brick_log2;
+    const int B = 1 << lb;
+    hvr_volume *v = new hvr_volume();
+    memset(v, 0, sizeof(*v));
+    v->ctx = c;
+    v->s0 = s0;
+    v->s1 = s1;
+    v->s2 = s2;
+    v->fmt = fmt;
+    v->lb = lb;
+    v->nbx = (s0 + B - 1) / B;
+    v->nby = (s1 + B - 1) / B;
+    v->nbz = (s2 + B - 1) / B;
+    v->ntexels = (uint64_t)v->nbx * v->nby * v->nbz << (3 * lb);
+    if (v->ntexels >= (1ull << 32)) {
+        delete v;
+        return fail(HVR_EINVAL, "volume too large (texel index exceeds 32 bits)");
+    }
+    const size_t bytes = (size_t)v->ntexels * texel_bytes(fmt);
+    hipError_t e = hipSetDevice(c->device);
+    if (e == hipSuccess) e = hipMalloc(&v->d_texels, bytes);
+    if (e == hipSuccess) e = hipMemsetAsync(v->d_texels, 0, bytes, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(v->d_texels);
+        delete v;
+        return fail_hip(e, "hvr_volume_create");
+    }
+    *out = v;
+    return HVR_OK;
+}
+
+static int upload_part(hvr_volume *v, const float *data, int part)
+{
+    hvr_ctx *c = v->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    const int comps = part == 1 ? 1 : 3;
+    const uint64_t plane = (uint64_t)v->s0 * v->s1;
+    // stage whole z-planes, up to 256 MiB per chunk
+    const size_t max_stage = (size_t)256 << 20;
+    uint64_t zper = max_stage / (plane * comps * sizeof(float));
+    if (zper < 1) zper = 1;
+    if (zper > (uint64_t)v->s2) zper = v->s2;
+    const size_t need = (size_t)(zper * plane * comps * sizeof(float));
+    if (c->stage_bytes < need) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        (void)hipFree(c->d_stage);
+        c->d_stage = nullptr;
+        c->stage_bytes = 0;
+        HIPCHK(hipMalloc(&c->d_stage, need));
+        c->stage_bytes = need;
+    }
+    const TexDesc t = tex_desc(v);
+    for (uint64_t z0 = 0; z0 < (uint64_t)v->s2; z0 += zper) {
+        const uint64_t nz = (z0 + zper <= (uint64_t)v->s2) ? zper : v->s2 - z0;
+        const uint64_t nvox = nz * plane;
+        HIPCHK(hipMemcpyAsync(c->d_stage, data + z0 * plane * comps, nvox * comps * sizeof(float),
+                              hipMemcpyHostToDevice, c->stream));
+        const unsigned nblk = (unsigned)((nvox + 255) / 256);
+        hipLaunchKernelGGL(k_upload, dim3(nblk), dim3(256), 0, c->stream, v->d_texels, v->fmt, v->s0,
+                           v->s1, v->lb, t.bsy, t.bsz, (const float *)c->d_stage, part, (int)z0, nvox);
+        HIPCHK(hipGetLastError());
+        // the staging buffer is reused by the next chunk
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    return HVR_OK;
+}
+
+int hvr_volume_upload_scalar(hvr_volume *v, const float *data)
+{
+    if (!v || !data) return fail(HVR_EINVAL, "null argument");
+    if (!fmt_has_scalar(v->fmt)) return fail(HVR_ESTATE, "volume has no scalar field");
+    int rc = upload_part(v, data, 1);
+    if (rc == HVR_OK) v->scalar_loaded = true;
+    return rc;
+}
+
+int hvr_volume_upload_normals(hvr_volume *v, const float *data)
+{
+    if (!v || !data) return fail(HVR_EINVAL, "null argument");
+    if (!fmt_has_normal(v->fmt)) return fail(HVR_ESTATE, "volume has no normal field");
+    int rc = upload_part(v, data, 2);
+    if (rc == HVR_OK) v->normals_loaded = true;
+    return rc;
+}
+
+int hvr_volume_bytes(const hvr_volume *v, uint64_t *bytes)
+{
+    if (!v || !bytes) return fail(HVR_EINVAL, "null argument");
+    *bytes = v->ntexels * (uint64_t)texel_bytes(v->fmt);
+    return HVR_OK;
+}
+
+int hvr_volume_destroy(hvr_volume *v)
+{
+    if (!v) return HVR_OK;
+    hvr_ctx *c = v->ctx;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    if (c->scalar_vol == v) c->scalar_vol = nullptr;
+    if (c->normal_vol == v) c->normal_vol = nullptr;
+    (void)hipFree(v->d_texels);
+    delete v;
+    return HVR_OK;
+}
+
+int hvr_set_lut(hvr_ctx *c, const float *rgba, int n)
+{
+    if (!c) return fail(HVR_EINVAL, "ctx is null");
+    if (n < 0 || (n > 0 && !rgba)) return fail(HVR_EINVAL, "bad LUT");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const float unbound[4] = {0.f, 0.f, 0.f, 1.f};
+    const float *src = n > 0 ? rgba : unbound;
+    const int cnt = n > 0 ? n : 1;
+    if (cnt > c->lut_cap) {
+        (void)hipFree(c->d_lut);
+        c->d_lut = nullptr;
+        c->lut_cap = 0;
+        HIPCHK(hipMalloc(&c->d_lut, (size_t)cnt * sizeof(float4)));
+        c->lut_cap = cnt;
+    }
+    HIPCHK(hipMemcpy(c->d_lut, src, (size_t)cnt * sizeof(float4), hipMemcpyHostToDevice));
+    c->lut_n = cnt;
+    return HVR_OK;
+}
+
+int hvr_set_params(hvr_ctx *c, const hvr_params *p)
+{
+    if (!c || !p) return fail(HVR_EINVAL, "null argument");
+    if (!(p->step > 0.f) || !(p->ref_step > 0.f)) return fail(HVR_EINVAL, "step sizes must be positive");
+    if (p->max_steps < 0) return fail(HVR_EINVAL, "max_steps must be >= 0");
+    c->params = *p;
+    c->params_set = true;
+    return HVR_OK;
+}
+
+int hvr_bind(hvr_ctx *c, hvr_volume *s, int srole, hvr_volume *n, int nrole)
+{
+    if (!c) return fail(HVR_EINVAL, "ctx is null");
+    if ((s && s->ctx != c) || (n && n->ctx != c)) return fail(HVR_EINVAL, "volume belongs to another context");
+    if (srole != HVR_ROLE_SCALAR && srole != HVR_ROLE_NORMAL) return fail(HVR_EINVAL, "bad scalar role");
+    if (nrole != HVR_ROLE_SCALAR && nrole != HVR_ROLE_NORMAL) return fail(HVR_EINVAL, "bad normal role");
+    for (int k = 0; k < 2; ++k) {
+        const hvr_volume *v = k ? n : s;
+        const int role = k ? nrole : srole;
+        if (!v) continue;
+        if (role == HVR_ROLE_SCALAR && !fmt_has_scalar(v->fmt)) return fail(HVR_EINVAL, "volume has no scalar field");
+        if (role == HVR_ROLE_NORMAL && !fmt_has_normal(v->fmt)) return fail(HVR_EINVAL, "volume has no normal field");
+    }
+    c->scalar_vol = s;
+    c->normal_vol = n;
+    c->scalar_role = srole;
+    c->normal_role = nrole;
+    return HVR_OK;
+}
+
+int hvr_render_async(hvr_ctx *c)
+{
+    if (!c) return fail(HVR_EINVAL, "ctx is null");
+    HIPCHK(hipSetDevice(c->device));
+    return enqueue_frame(c, false);
+}
+
+int hvr_read(hvr_ctx *c, uint8_t *out)
+{
+    if (!c || !out) return fail(HVR_EINVAL, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    const size_t bytes = (size_t)c->W * c->H * 4;
+    HIPCHK(hipMemcpyAsync(c->h_pinned, c->d_out, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    memcpy(out, c->h_pinned, bytes);
+    return HVR_OK;
+}
+
+int hvr_render(hvr_ctx *c, uint8_t *out)
+{
+    int rc = hvr_render_async(c);
+    if (rc != HVR_OK) return rc;
+    return hvr_read(c, out);
+}
+
+int hvr_synchronize(hvr_ctx *c)
+{
+    if (!c) return fail(HVR_EINVAL, "ctx is null");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return HVR_OK;
+}
+
+int hvr_framebuffer(hvr_ctx *c, void **ptr)
+{
+    if (!c || !ptr) return fail(HVR_EINVAL, "null argument");
+    *ptr = c->d_out;
+    return HVR_OK;
+}
+
+int hvr_sample_counts(hvr_ctx *c, uint32_t *counts)
+{
+    if (!c || !counts) return fail(HVR_EINVAL, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    const size_t npix = (size_t)c->W * c->H;
+    if (!c->d_counts) HIPCHK(hipMalloc(&c->d_counts, npix * sizeof(uint32_t)));
+    int rc = enqueue_frame(c, true);
+    if (rc != HVR_OK) return rc;
+    HIPCHK(hipMemcpyAsync(counts, c->d_counts, npix * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return HVR_OK;
+}
+
+int hvr_last_kernel_ms(hvr_ctx *c, float *march_ms, float *setup_ms)
+{
+    if (!c) return fail(HVR_EINVAL, "ctx is null");
+    if (!c->events_recorded) return fail(HVR_ESTATE, "no frame rendered yet");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipEventSynchronize(c->ev[2]));
+    float a = 0.f, b = 0.f;
+    HIPCHK(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+    HIPCHK(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+    if (setup_ms) *setup_ms = a;
+    if (march_ms) *march_ms = b;
+    return HVR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,20 @@
+"""HIP renderer: ``VolumeRenderer`` (drop-in for the reference's
+``pyvr.moderngl_renderer.VolumeRenderer``) and its device-resource manager."""
+
+from ..camera import Camera
+from ..datasets import compute_normal_volume, create_sample_volume
+from ..transferfunctions.color import ColorTransferFunction
+from ..transferfunctions.opacity import OpacityTransferFunction
+from .manager import HipManager
+from .renderer import HipVolumeRenderer, VolumeRenderer
+
+__all__ = [
+    "ColorTransferFunction",
+    "OpacityTransferFunction",
+    "HipVolumeRenderer",
+    "VolumeRenderer",
+    "HipManager",
+    "Camera",
+    "create_sample_volume",
+    "compute_normal_volume",
+]

@@ -1,0 +1,111 @@
+"""ctypes binding of ``libhipvr.so`` (declarations in ``include/hipvr.h``).
+
+The library is built in-tree by ``pyvr_amd.build`` (hipcc, gfx950).  There is
+no fallback: if the library is missing or cannot be loaded, every renderer
+call raises ``RuntimeError``.
+"""
+
+import ctypes
+import os
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "libhipvr.so")
+
+HVR_OK = 0
+HVR_SCALAR = {"none": 0, "f32": 1, "f16": 2, "u8": 3}
+HVR_NORMAL = {"none": 0, "f32x3": 1, "sn8x3": 2}
+HVR_ROLE_SCALAR = 0
+HVR_ROLE_NORMAL = 1
+
+# exported symbols, as declared in include/hipvr.h
+EXPORTS = (
+    "hvr_abi_version", "hvr_last_error", "hvr_device_count", "hvr_create", "hvr_destroy",
+    "hvr_volume_create", "hvr_volume_upload_scalar", "hvr_volume_upload_normals",
+    "hvr_volume_bytes", "hvr_volume_destroy", "hvr_set_lut", "hvr_set_params", "hvr_bind",
+    "hvr_render", "hvr_render_async", "hvr_read", "hvr_synchronize", "hvr_framebuffer",
+    "hvr_sample_counts", "hvr_last_kernel_ms", "hvr_set_brick_log2",
+)
+
+
+class HvrParams(ctypes.Structure):
+    _fields_ = [
+        ("view", ctypes.c_float * 16),
+        ("proj", ctypes.c_float * 16),
+        ("cam_pos", ctypes.c_float * 3),
+        ("bmin", ctypes.c_float * 3),
+        ("bmax", ctypes.c_float * 3),
+        ("step", ctypes.c_float),
+        ("ref_step", ctypes.c_float),
+        ("max_steps", ctypes.c_int32),
+        ("ambient", ctypes.c_float),
+        ("diffuse", ctypes.c_float),
+        ("light_pos", ctypes.c_float * 3),
+        ("light_target", ctypes.c_float * 3),
+        ("et_threshold", ctypes.c_float),
+        ("camera_set", ctypes.c_int32),
+        ("clear_rgba", ctypes.c_uint8 * 4),
+        ("blend", ctypes.c_int32),
+    ]
+
+
+_lib = None
+_load_error = None
+
+
+def _declare(L):
+    vp, i, u8p = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint8)
+    fp = ctypes.POINTER(ctypes.c_float)
+    sig = {
+        "hvr_abi_version": ([], i),
+        "hvr_last_error": ([], ctypes.c_char_p),
+        "hvr_device_count": ([ctypes.POINTER(i)], i),
+        "hvr_create": ([i, i, i, ctypes.POINTER(vp)], i),
+        "hvr_destroy": ([vp], i),
+        "hvr_volume_create": ([vp, i, i, i, i, i, ctypes.POINTER(vp)], i),
+        "hvr_volume_upload_scalar": ([vp, fp], i),
+        "hvr_volume_upload_normals": ([vp, fp], i),
+        "hvr_volume_bytes": ([vp, ctypes.POINTER(ctypes.c_uint64)], i),
+        "hvr_volume_destroy": ([vp], i),
+        "hvr_set_lut": ([vp, fp, i], i),
+        "hvr_set_params": ([vp, ctypes.POINTER(HvrParams)], i),
+        "hvr_bind": ([vp, vp, i, vp, i], i),
+        "hvr_render": ([vp, u8p], i),
+        "hvr_render_async": ([vp], i),
+        "hvr_read": ([vp, u8p], i),
+        "hvr_synchronize": ([vp], i),
+        "hvr_framebuffer": ([vp, ctypes.POINTER(vp)], i),
+        "hvr_sample_counts": ([vp, ctypes.POINTER(ctypes.c_uint32)], i),
+        "hvr_last_kernel_ms": ([vp, fp, fp], i),
+        "hvr_set_brick_log2": ([vp, i], i),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+
+
+def lib():
+    """The loaded library; raises RuntimeError when it is not available."""
+    global _lib, _load_error
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"libhipvr.so not found at {LIB_PATH}; build it with "
+            "`python -c 'import pyvr_amd; pyvr_amd.build()'` (needs hipcc)"
+        )
+    try:
+        L = ctypes.CDLL(LIB_PATH)
+        _declare(L)
+    except OSError as e:  # missing HIP runtime etc.
+        _load_error = e
+        raise RuntimeError(f"cannot load {LIB_PATH}: {e}") from e
+    _lib = L
+    return _lib
+
+
+def check(rc: int, what: str = "") -> None:
+    """Raise RuntimeError carrying the library's message for a failed call."""
+    if rc != HVR_OK:
+        msg = lib().hvr_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed ({rc}): {msg}" if what else f"hipvr error {rc}: {msg}")
