@@ -1,0 +1,288 @@
+#!/usr/bin/env python3
+"""Benchmark of the HIP volume renderer on one MI355X.
+
+Metric: ray samples/s (plus fps and p50/p99 frame latency) of the plain
+one-request-per-launch renderer.  A *step* renders the workload's camera set
+once (default: 24 cameras of the 512^3 blob volume at 1024x1024, high_quality
+preset, FP16 scalar + SNORM8x3 normals), each frame into its own resident
+device buffer, with the volume already resident in HBM.
+
+* samples: reference-algorithm samples (pixel, i) -- i < max_steps, alpha
+  below the termination threshold, sample inside the box -- counted per
+  camera by the renderer's counting pass before the timed region (the tests
+  pin that count to the oracle's);
+* value = samples per step * steps / wall time of the timed steps;
+* roofline: the march kernel, HBM-bound; algorithmic bytes per launch =
+  samples * 8 * (scalar + normal bytes per voxel, unpadded) + 4 * W * H,
+  divided by the kernel's average duration from HIP events recorded on its
+  stream over the timed region;
+* fps / latency: end-to-end VolumeRenderer.render() (set_camera + draw +
+  copy of the RGBA8 image to the host), per frame;
+* cpu_baseline: the oracle's FP32 port of the reference shader (its loop
+  runs every i < max_steps, as the shader does) on all host cores, whole
+  frames of the workload's cameras until ~15 s have elapsed.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+SCALAR_BYTES = {"f32": 4, "f16": 2, "u8": 1}
+NORMAL_BYTES = {"f32x3": 12, "sn8x3": 3, "none": 0}
+METRIC = "ray samples/s + fps at 512^3/1024^2; p50/p99 frame latency vs plain kernel"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_workload(args):
+    import scenes
+
+    if args.workload == "c2":
+        sc = scenes.c2(size=args.size or 512, image=args.image or 1024)
+        cams = scenes.c2_cameras(args.cameras)
+    elif args.workload == "c1":
+        sc = scenes.c1(size=args.size or 256, image=args.image or 1024)
+        cams = [sc.camera] * args.cameras
+    elif args.workload == "c0":
+        sc = scenes.c0(size=args.size or 128, image=args.image or 512)
+        cams = [sc.camera] * args.cameras
+    else:
+        raise SystemExit(f"unknown workload {args.workload}")
+    return sc, cams
+
+
+def kernel_source_hash() -> str:
+    """Hash of the device sources: measured traffic is only reported for the
+    kernels it was measured on."""
+    import hashlib
+
+    h = hashlib.sha256()
+    for rel in ("pyvr_amd/csrc/hvr_device.h", "pyvr_amd/csrc/hipvr.hip"):
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def measured_traffic(key: str):
+    """HBM bytes per march launch from the rocprofv3 FETCH_SIZE/WRITE_SIZE
+    passes recorded in profiles/traffic.json (scaled by the copy-kernel
+    calibration), or None when no measurement matches this workload and
+    these kernel sources."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return None, None
+    e = table.get(key)
+    if not e or e.get("kernel_sources") != kernel_source_hash():
+        return None, None
+    return e["read_bytes"] + e["write_bytes"], e.get("source")
+
+
+def cpu_baseline(sc, cams, args, sf, nf):
+    """Oracle FP32 port of the shader (reference loop to max_steps) on all
+    host cores: whole frames of the workload's cameras, in order, until
+    ~cpu_seconds have elapsed."""
+    import oracle
+    import scenes
+
+    oracle.build()
+    cores = len(os.sched_getaffinity(0))
+    data = scenes.dequant_scalar(sc.data, sf)
+    nrm = scenes.dequant_normals(sc.normals, nf) if sc.normals is not None else data
+    lut = sc.lut()
+    samples, frames, t0 = 0, 0, time.perf_counter()
+    while time.perf_counter() - t0 < args.cpu_seconds:
+        u = sc.uniforms(cams[frames % len(cams)])
+        out = oracle.render(u, data, nrm, lut, mode="f32", full_loop=True, nthreads=cores,
+                            want=("counts",))
+        samples += int(out["counts"].sum())
+        frames += 1
+    dt = time.perf_counter() - t0
+    log(f"cpu baseline: {frames} frames, {samples} samples in {dt:.1f}s on {cores} threads")
+    return {
+        "value": samples / dt, "unit": "samples/s", "cores": cores, "kind": "port",
+        "sample": (f"{frames} whole {sc.width}x{sc.height} frames (cameras 0..{frames - 1} of the "
+                   f"workload, cycling), FP32 shader arithmetic, reference loop to max_steps, "
+                   f"{samples} samples in {dt:.1f}s; Mesa llvmpipe is not available on the host "
+                   "(no EGL/OSMesa/X server), so this C port of the shader stands in for it"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c2", choices=["c0", "c1", "c2"])
+    ap.add_argument("--size", type=int, default=0)
+    ap.add_argument("--image", type=int, default=0)
+    ap.add_argument("--cameras", type=int, default=24)
+    ap.add_argument("--scalar-format", default="f16", choices=["f32", "f16", "u8"])
+    ap.add_argument("--normal-format", default="sn8x3", choices=["f32x3", "sn8x3"])
+    ap.add_argument("--brick-log2", type=int, default=-1)
+    ap.add_argument("--latency-frames", type=int, default=48)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="before the timed region, copy this many MiB device-to-device once "
+                         "(a known-size kernel for calibrating traffic counters)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); this renderer's "
+              "multi-GPU split is a design note, not a measured path.", file=sys.stderr)
+        sys.exit(2)
+
+    from pyvr_amd.hip_renderer import VolumeRenderer, _lib
+
+    t0 = time.perf_counter()
+    sc, cams = make_workload(args)
+    sf, nf = args.scalar_format, args.normal_format
+    log(f"workload {sc.name}: generated in {time.perf_counter() - t0:.1f}s")
+
+    r = VolumeRenderer(sc.width, sc.height, config=sc.config, light=sc.light,
+                       scalar_format=sf, normal_format=nf)
+    L = _lib.lib()
+    ctx = r.gl_manager.ctx
+    if args.brick_log2 >= 0:
+        _lib.check(L.hvr_set_brick_log2(ctx, args.brick_log2), "set_brick")
+    t0 = time.perf_counter()
+    r.load_volume(sc.volume())
+    vol_bytes = sum(v.nbytes() for v in r.gl_manager._volumes)
+    log(f"volume resident: {vol_bytes / 2**20:.0f} MiB in {time.perf_counter() - t0:.1f}s")
+    r.set_transfer_functions(sc.ctf, sc.otf, sc.lut_size)
+    r.set_opacity_threshold(sc.tau)
+
+    # per-camera parameter blocks, exactly as set_camera + render would set them
+    params, counts = [], []
+    for cam in cams:
+        r.set_camera(cam)
+        r.gl_manager.clear_framebuffer(0, 0, 0, 0)
+        r.gl_manager.setup_blending()
+        r.gl_manager._sync_bindings()
+        params.append(r.gl_manager._params())
+        counts.append(int(r.sample_counts().sum()))
+    s_step = sum(counts)
+    npix = sc.width * sc.height
+    log(f"samples per step: {s_step} ({len(cams)} frames)")
+
+    bufs = []
+    for _ in cams:
+        p = ctypes.c_void_p()
+        _lib.check(L.hvr_device_alloc(ctx, npix * 4, ctypes.byref(p)), "alloc")
+        bufs.append(p)
+
+    def step():
+        for p, b in zip(params, bufs):
+            _lib.check(L.hvr_set_params(ctx, ctypes.byref(p)), "set_params")
+            _lib.check(L.hvr_render_async_to(ctx, b), "render")
+
+    if args.copy_calibration_mib > 0:
+        nb = args.copy_calibration_mib << 20
+        a, b = ctypes.c_void_p(), ctypes.c_void_p()
+        _lib.check(L.hvr_device_alloc(ctx, nb, ctypes.byref(a)), "alloc")
+        _lib.check(L.hvr_device_alloc(ctx, nb, ctypes.byref(b)), "alloc")
+        _lib.check(L.hvr_device_copy(ctx, b, a, nb), "copy")
+        L.hvr_device_free(ctx, a)
+        L.hvr_device_free(ctx, b)
+        log(f"calibration copy: {args.copy_calibration_mib} MiB device-to-device")
+
+    for _ in range(args.warmup):
+        step()
+    _lib.check(L.hvr_synchronize(ctx), "sync")
+    _lib.check(L.hvr_timing_begin(ctx, args.steps * len(cams)), "timing")
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    _lib.check(L.hvr_synchronize(ctx), "sync")
+    wall = time.perf_counter() - t_start
+    march_ms, setup_ms, nfr = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+    _lib.check(L.hvr_timing_end(ctx, ctypes.byref(march_ms), ctypes.byref(setup_ms), ctypes.byref(nfr)),
+               "timing_end")
+    value = s_step * args.steps / wall
+    ms_step = wall * 1e3 / args.steps
+    log(f"timed {args.steps} steps: {ms_step:.2f} ms/step, {value / 1e9:.2f} G samples/s; "
+        f"march kernel avg {march_ms.value / nfr.value:.3f} ms, setup {setup_ms.value / nfr.value:.3f} ms")
+
+    bpv = 8 * (SCALAR_BYTES[sf] + (NORMAL_BYTES[nf] if sc.normals is not None else 0))
+    alg_bytes = s_step * args.steps * bpv + 4 * npix * nfr.value
+    achieved = alg_bytes / (march_ms.value * 1e-3)
+    tkey = f"{sc.name}/{len(cams)}cams/{sf}+{nf if sc.normals is not None else 'none'}/lb{args.brick_log2}"
+    traffic, tsrc = measured_traffic(tkey)
+    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK, "traffic": traffic,
+                "traffic_source": tsrc, "traffic_key": tkey,
+                "alg_bytes_per_launch": alg_bytes / nfr.value,
+                "bytes_per_sample": bpv, "kernel": "hvr::k_march",
+                "avg_kernel_ms": march_ms.value / nfr.value}
+
+    # end-to-end frames through the public API (set_camera + render + D2H)
+    lat = []
+    for i in range(args.latency_frames + 2):
+        cam = cams[i % len(cams)]
+        t = time.perf_counter()
+        r.set_camera(cam)
+        r.render()
+        if i >= 2:
+            lat.append((time.perf_counter() - t) * 1e3)
+    lat = np.array(lat)
+
+    for b in bufs:
+        L.hvr_device_free(ctx, b)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(sc, cams, args, sf, nf)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{sc.name}: {len(cams)} cameras, {sc.width}x{sc.height}, "
+                        f"{sc.data.shape[0]}^3 blob volume" if args.workload != "c0" else sc.name,
+            "volume": list(sc.data.shape), "image": [sc.width, sc.height],
+            "cameras": len(cams), "preset": {"step_size": sc.config.step_size,
+                                             "max_steps": sc.config.max_steps},
+            "et_threshold": sc.tau, "scalar_format": sf,
+            "normal_format": nf if sc.normals is not None else "none",
+            "samples_per_step": s_step, "volume_bytes": vol_bytes,
+        },
+        "fps": 1e3 / float(lat.mean()),
+        "frame_ms_p50": float(np.percentile(lat, 50)),
+        "frame_ms_p99": float(np.percentile(lat, 99)),
+        "frames_per_s_device": len(cams) * args.steps / wall,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    r.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -160,6 +160,27 @@ int hvr_sample_counts(hvr_ctx *ctx, uint32_t *counts);
  * measured with HIP events on the context's stream. */
 int hvr_last_kernel_ms(hvr_ctx *ctx, float *march_ms, float *setup_ms);
 
+/* Render into a caller-provided device buffer of width*height*4 bytes
+ * instead of the context's framebuffer (no counterpart in the reference:
+ * lets a caller keep many frames resident).  NULL means the framebuffer. */
+int hvr_render_async_to(hvr_ctx *ctx, void *device_rgba8);
+
+/* Device memory owned by the caller (for hvr_render_async_to). */
+int hvr_device_alloc(hvr_ctx *ctx, uint64_t bytes, void **device_ptr);
+int hvr_device_free(hvr_ctx *ctx, void *device_ptr);
+/* Copy `bytes` from device memory to host memory (waits). */
+int hvr_copy_to_host(hvr_ctx *ctx, void *host, const void *device_ptr, uint64_t bytes);
+/* Device-to-device copy on the context's stream (waits); a plain copy of
+ * known size, used to calibrate hardware traffic counters. */
+int hvr_device_copy(hvr_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+
+/* Per-launch kernel timing over a region: after hvr_timing_begin every
+ * frame records HIP events around its ray-setup and march kernels on the
+ * context's stream (up to `max_frames` frames); hvr_timing_end waits and
+ * returns the summed kernel milliseconds and the number of frames timed. */
+int hvr_timing_begin(hvr_ctx *ctx, int max_frames);
+int hvr_timing_end(hvr_ctx *ctx, double *march_ms, double *setup_ms, int *frames);
+
 /* Tuning knobs (no counterpart in the reference): log2 of the brick edge
  * used by volumes created afterwards (0 = linear layout, 2 = 4^3 bricks,
  * 3 = 8^3 bricks). */

@@ -67,6 +67,9 @@ struct hvr_ctx {
     hvr_volume *scalar_vol, *normal_vol;
     int scalar_role, normal_role;
     int brick_log2;
+    // per-launch timing region (hvr_timing_begin/end)
+    std::vector<hipEvent_t> *tev;
+    int timing_max, timing_n;
 };
 
 static const int kLutLdsMax = 4096;  // 64 KiB of LDS per block
@@ -234,18 +237,33 @@ static void launch_march(hvr_ctx *c, const FrameConst &fc)
     launch_march_t<TF_NONE, NM_ANY, COUNT>(c, fc, ts, tn, ntx, nblocks);
 }
 
-static int enqueue_frame(hvr_ctx *c, bool count)
+static int enqueue_frame(hvr_ctx *c, bool count, uchar4 *dst)
 {
     const FrameConst fc = frame_const(c);
     const int npix = c->W * c->H;
-    HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    hipEvent_t e0 = c->ev[0], e1 = c->ev[1], e2 = c->ev[2];
+    if (c->tev && c->timing_n < c->timing_max) {
+        const size_t k = 3 * (size_t)c->timing_n++;
+        e0 = (*c->tev)[k];
+        e1 = (*c->tev)[k + 1];
+        e2 = (*c->tev)[k + 2];
+    }
+    uchar4 *saved = c->d_out;
+    if (dst) c->d_out = dst;
+    HIPCHK(hipEventRecord(e0, c->stream));
     hipLaunchKernelGGL(k_ray_setup, dim3((npix + 255) / 256), dim3(256), 0, c->stream, fc, c->d_rays);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(c->ev[1], c->stream));
+    HIPCHK(hipEventRecord(e1, c->stream));
     if (count) launch_march<true>(c, fc);
     else launch_march<false>(c, fc);
+    c->d_out = saved;
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(c->ev[2], c->stream));
+    HIPCHK(hipEventRecord(e2, c->stream));
+    if (e2 != c->ev[2]) {  // keep hvr_last_kernel_ms meaningful
+        HIPCHK(hipEventRecord(c->ev[0], c->stream));
+        HIPCHK(hipEventRecord(c->ev[1], c->stream));
+        HIPCHK(hipEventRecord(c->ev[2], c->stream));
+    }
     c->events_recorded = true;
     return HVR_OK;
 }
@@ -314,6 +332,10 @@ int hvr_destroy(hvr_ctx *c)
     if (c->h_pinned) (void)hipHostFree(c->h_pinned);
     for (int i = 0; i < 4; ++i)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    if (c->tev) {
+        for (hipEvent_t e : *c->tev) (void)hipEventDestroy(e);
+        delete c->tev;
+    }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return HVR_OK;
@@ -519,7 +541,87 @@ int hvr_render_async(hvr_ctx *c)
 {
     if (!c) return fail(HVR_EINVAL, "ctx is null");
     HIPCHK(hipSetDevice(c->device));
-    return enqueue_frame(c, false);
+    return enqueue_frame(c, false, nullptr);
+}
+
+int hvr_render_async_to(hvr_ctx *c, void *dst)
+{
+    if (!c) return fail(HVR_EINVAL, "ctx is null");
+    HIPCHK(hipSetDevice(c->device));
+    return enqueue_frame(c, false, (uchar4 *)dst);
+}
+
+int hvr_device_alloc(hvr_ctx *c, uint64_t bytes, void **ptr)
+{
+    if (!c || !ptr) return fail(HVR_EINVAL, "null argument");
+    *ptr = nullptr;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMalloc(ptr, bytes ? bytes : 1));
+    return HVR_OK;
+}
+
+int hvr_device_free(hvr_ctx *c, void *ptr)
+{
+    if (!c) return fail(HVR_EINVAL, "ctx is null");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipFree(ptr));
+    return HVR_OK;
+}
+
+int hvr_copy_to_host(hvr_ctx *c, void *host, const void *dev, uint64_t bytes)
+{
+    if (!c || !host || !dev) return fail(HVR_EINVAL, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return HVR_OK;
+}
+
+int hvr_device_copy(hvr_ctx *c, void *dst, const void *src, uint64_t bytes)
+{
+    if (!c || !dst || !src) return fail(HVR_EINVAL, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return HVR_OK;
+}
+
+int hvr_timing_begin(hvr_ctx *c, int max_frames)
+{
+    if (!c) return fail(HVR_EINVAL, "ctx is null");
+    if (max_frames <= 0 || max_frames > (1 << 16)) return fail(HVR_EINVAL, "max_frames out of range");
+    HIPCHK(hipSetDevice(c->device));
+    if (!c->tev) c->tev = new std::vector<hipEvent_t>();
+    while (c->tev->size() < 3 * (size_t)max_frames) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        c->tev->push_back(e);
+    }
+    c->timing_max = max_frames;
+    c->timing_n = 0;
+    return HVR_OK;
+}
+
+int hvr_timing_end(hvr_ctx *c, double *march_ms, double *setup_ms, int *frames)
+{
+    if (!c || !c->tev) return fail(HVR_ESTATE, "no timing region");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double m = 0.0, s = 0.0;
+    for (int i = 0; i < c->timing_n; ++i) {
+        float a = 0.f, b = 0.f;
+        HIPCHK(hipEventElapsedTime(&a, (*c->tev)[3 * i], (*c->tev)[3 * i + 1]));
+        HIPCHK(hipEventElapsedTime(&b, (*c->tev)[3 * i + 1], (*c->tev)[3 * i + 2]));
+        s += a;
+        m += b;
+    }
+    if (march_ms) *march_ms = m;
+    if (setup_ms) *setup_ms = s;
+    if (frames) *frames = c->timing_n;
+    c->timing_max = 0;
+    c->timing_n = 0;
+    return HVR_OK;
 }
 
 int hvr_read(hvr_ctx *c, uint8_t *out)
@@ -561,7 +663,7 @@ int hvr_sample_counts(hvr_ctx *c, uint32_t *counts)
     HIPCHK(hipSetDevice(c->device));
     const size_t npix = (size_t)c->W * c->H;
     if (!c->d_counts) HIPCHK(hipMalloc(&c->d_counts, npix * sizeof(uint32_t)));
-    int rc = enqueue_frame(c, true);
+    int rc = enqueue_frame(c, true, nullptr);
     if (rc != HVR_OK) return rc;
     HIPCHK(hipMemcpyAsync(counts, c->d_counts, npix * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));

@@ -24,6 +24,8 @@ EXPORTS = (
     "hvr_volume_bytes", "hvr_volume_destroy", "hvr_set_lut", "hvr_set_params", "hvr_bind",
     "hvr_render", "hvr_render_async", "hvr_read", "hvr_synchronize", "hvr_framebuffer",
     "hvr_sample_counts", "hvr_last_kernel_ms", "hvr_set_brick_log2",
+    "hvr_render_async_to", "hvr_device_alloc", "hvr_device_free", "hvr_copy_to_host",
+    "hvr_timing_begin", "hvr_timing_end", "hvr_device_copy",
 )
 
 
@@ -77,6 +79,14 @@ def _declare(L):
         "hvr_sample_counts": ([vp, ctypes.POINTER(ctypes.c_uint32)], i),
         "hvr_last_kernel_ms": ([vp, fp, fp], i),
         "hvr_set_brick_log2": ([vp, i], i),
+        "hvr_render_async_to": ([vp, vp], i),
+        "hvr_device_alloc": ([vp, ctypes.c_uint64, ctypes.POINTER(vp)], i),
+        "hvr_device_free": ([vp, vp], i),
+        "hvr_copy_to_host": ([vp, vp, vp, ctypes.c_uint64], i),
+        "hvr_device_copy": ([vp, vp, vp, ctypes.c_uint64], i),
+        "hvr_timing_begin": ([vp, i], i),
+        "hvr_timing_end": ([vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                            ctypes.POINTER(i)], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -84,11 +84,15 @@ def standard_light() -> Light:
 
 
 def seeded_otf(n_points: int, seed: int, y_max: float = 0.5) -> OpacityTransferFunction:
-    """Piecewise-linear OTF with control points x = [0, sorted U(0,1)^(n-2), 1],
-    y ~ U(0, y_max), drawn from ``default_rng(100 + seed)``."""
+    """Piecewise-linear OTF with control points x = [0, sorted U(0,1)^(n-2), 1]
+    and y = [0, U(0, y_max)^(n-1)], drawn from ``default_rng(100 + seed)``.
+
+    Density 0 (empty space) is transparent.  With a random opacity at 0 the
+    whole box is a fog and every ray reaches the termination alpha within
+    ~20 samples of the box face, whatever the volume holds."""
     rng = np.random.default_rng(100 + seed)
     xs = np.concatenate([[0.0], np.sort(rng.uniform(0.0, 1.0, n_points - 2)), [1.0]])
-    ys = rng.uniform(0.0, y_max, n_points)
+    ys = np.concatenate([[0.0], rng.uniform(0.0, y_max, n_points - 1)])
     return OpacityTransferFunction([(float(x), float(y)) for x, y in zip(xs, ys)])
 
 
