@@ -77,37 +77,37 @@ class _MarchProgram:
 class _DeviceVolume:
     """One device volume (``hvr_volume``): scalar and/or normal field."""
 
-    def __init__(self, ctx, shape, scalar_format, normal_format):
+    def __init__(self, L, ctx, shape, scalar_format, normal_format):
+        self.L = L
         self.shape = tuple(int(s) for s in shape)
         self.scalar_format = scalar_format
         self.normal_format = normal_format
         h = ctypes.c_void_p()
-        check(_lib.lib().hvr_volume_create(ctx, self.shape[0], self.shape[1], self.shape[2],
-                                           _lib.HVR_SCALAR[scalar_format],
-                                           _lib.HVR_NORMAL[normal_format], ctypes.byref(h)),
-              "hvr_volume_create")
+        check(L.hvr_volume_create(ctx, self.shape[0], self.shape[1], self.shape[2],
+                                  _lib.HVR_SCALAR[scalar_format], _lib.HVR_NORMAL[normal_format],
+                                  ctypes.byref(h)), "hvr_volume_create")
         self.handle = h
         self.normals_loaded = False
 
     def upload_scalar(self, data: np.ndarray):
         a = np.ascontiguousarray(data, dtype=np.float32)
-        check(_lib.lib().hvr_volume_upload_scalar(self.handle, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+        check(self.L.hvr_volume_upload_scalar(self.handle, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
               "hvr_volume_upload_scalar")
 
     def upload_normals(self, data: np.ndarray):
         a = np.ascontiguousarray(data, dtype=np.float32)
-        check(_lib.lib().hvr_volume_upload_normals(self.handle, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+        check(self.L.hvr_volume_upload_normals(self.handle, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
               "hvr_volume_upload_normals")
         self.normals_loaded = True
 
     def nbytes(self) -> int:
         v = ctypes.c_uint64()
-        check(_lib.lib().hvr_volume_bytes(self.handle, ctypes.byref(v)), "hvr_volume_bytes")
+        check(self.L.hvr_volume_bytes(self.handle, ctypes.byref(v)), "hvr_volume_bytes")
         return int(v.value)
 
     def destroy(self):
         if self.handle:
-            _lib.lib().hvr_volume_destroy(self.handle)
+            self.L.hvr_volume_destroy(self.handle)
             self.handle = None
 
 
@@ -138,8 +138,10 @@ class HipManager:
         self.width = width
         self.height = height
         self.device = device
+        self._released = True
+        self.L = _lib.lib()  # the library this context lives in
         h = ctypes.c_void_p()
-        check(_lib.lib().hvr_create(int(device), int(width), int(height), ctypes.byref(h)), "hvr_create")
+        check(self.L.hvr_create(int(device), int(width), int(height), ctypes.byref(h)), "hvr_create")
         self.ctx = h
         self.program = None
         self._next_texture_unit = 0
@@ -190,7 +192,7 @@ class HipManager:
         nf = _FORMAT_ALIASES.get(normal_format)
         if sf not in ("f32", "f16", "u8") or (nf not in (None, "f32x3", "sn8x3")):
             raise ValueError(f"unsupported formats scalar={scalar_format!r} normal={normal_format!r}")
-        dv = _DeviceVolume(self.ctx, volume_data.shape, sf, nf or "none")
+        dv = _DeviceVolume(self.L, self.ctx, volume_data.shape, sf, nf or "none")
         self._volumes.append(dv)
         dv.upload_scalar(np.asarray(volume_data, dtype=np.float32))
         unit = self._add_texture(_Texture3D(dv, _lib.HVR_ROLE_SCALAR, 1))
@@ -216,7 +218,7 @@ class HipManager:
                 and not newest.volume.normals_loaded):
             target = newest.volume           # interleave into the reserved texels
         if target is None:
-            target = _DeviceVolume(self.ctx, shape, "none", nf)
+            target = _DeviceVolume(self.L, self.ctx, shape, "none", nf)
             self._volumes.append(target)
         target.upload_normals(data)
         unit = self._add_texture(_Texture3D(target, _lib.HVR_ROLE_NORMAL, 3))
@@ -240,11 +242,16 @@ class HipManager:
     def _collect(self):
         """Release textures that no sampler references (keeping the newest)
         and device volumes that no texture references."""
+        # a sampler keeps only a texture of its own kind alive
+        keep = {self._newest_unit}
         if self.program is not None:
-            keep = {int(self.program.values[s]) for s in _SAMPLERS}
+            v = self.program.values
+            for name, kind in (("volume_texture", _Texture3D), ("normal_volume", _Texture3D),
+                               ("transfer_function_lut", _TextureLUT)):
+                if isinstance(self._units.get(int(v[name])), kind):
+                    keep.add(int(v[name]))
         else:
-            keep = {0}
-        keep.add(self._newest_unit)
+            keep.add(0)
         for unit in list(self._units):
             if unit not in keep:
                 del self._units[unit]
@@ -302,7 +309,7 @@ class HipManager:
         return t if isinstance(t, kind) else None
 
     def _sync_bindings(self):
-        L = _lib.lib()
+        L = self.L
         if self.program is None:
             check(L.hvr_bind(self.ctx, None, 0, None, 0), "hvr_bind")
             return
@@ -351,14 +358,14 @@ class HipManager:
         p = self._params()
         if not (p.step > 0 and p.ref_step > 0):
             raise RuntimeError("step_size and reference_step_size must be set and positive")
-        L = _lib.lib()
+        L = self.L
         check(L.hvr_set_params(self.ctx, ctypes.byref(p)), "hvr_set_params")
         check(L.hvr_render_async(self.ctx), "hvr_render_async")
 
     def read_pixels(self) -> bytes:
         """Framebuffer contents: width*height*4 bytes, RGBA8, bottom row first."""
         buf = (ctypes.c_uint8 * (self.width * self.height * 4))()
-        check(_lib.lib().hvr_read(self.ctx, buf), "hvr_read")
+        check(self.L.hvr_read(self.ctx, buf), "hvr_read")
         return bytes(buf)
 
     def sample_counts(self) -> np.ndarray:
@@ -366,7 +373,7 @@ class HipManager:
         bottom row first (a separate counting pass)."""
         self._sync_bindings()
         p = self._params()
-        L = _lib.lib()
+        L = self.L
         check(L.hvr_set_params(self.ctx, ctypes.byref(p)), "hvr_set_params")
         out = np.zeros((self.height, self.width), np.uint32)
         check(L.hvr_sample_counts(self.ctx, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))),
@@ -376,7 +383,7 @@ class HipManager:
     def last_kernel_ms(self):
         """(march_ms, setup_ms) of the last frame, from HIP events."""
         a, b = ctypes.c_float(), ctypes.c_float()
-        check(_lib.lib().hvr_last_kernel_ms(self.ctx, ctypes.byref(a), ctypes.byref(b)), "hvr_last_kernel_ms")
+        check(self.L.hvr_last_kernel_ms(self.ctx, ctypes.byref(a), ctypes.byref(b)), "hvr_last_kernel_ms")
         return a.value, b.value
 
     def cleanup(self):
@@ -389,7 +396,7 @@ class HipManager:
             dv.destroy()
         self._volumes = []
         if self.ctx:
-            _lib.lib().hvr_destroy(self.ctx)
+            self.L.hvr_destroy(self.ctx)
         self.ctx = None
         self._released = True
 

@@ -1,0 +1,164 @@
+"""Camera placement and matrices.
+
+Pins: the reference's known answers (tests/test_camera/test_control.py:22-112,
+test_matrix_creation.py:68-185), the closed form of the default placement
+(pos = d(cos el cos az, cos el sin az, sin el)), and scipy's Rotation (the
+library the reference builds rotations with) on random parameters."""
+
+import json
+
+import numpy as np
+import pytest
+from scipy.spatial.transform import Rotation
+
+from pyvr_amd.camera import (Camera, CameraError, degrees_to_radians, get_camera_pos,
+                             get_camera_pos_from_params, radians_to_degrees, validate_camera_angles)
+from pyvr_amd.camera.control import _quat_from_rotvec, _quat_mul, _quat_rotate
+
+
+def scipy_camera_pos(target, az, el, roll, dist, init_pos, init_up):
+    """The reference's placement rule written with scipy Rotation."""
+    target = target.astype(np.float32)
+    init_pos = init_pos.astype(np.float32)
+    init_up = init_up.astype(np.float32)
+    rel = init_pos - target
+    rel = rel / np.linalg.norm(rel) * dist
+    view = -rel / np.linalg.norm(rel)
+    axis = np.cross(init_up, view)
+    axis = axis / np.linalg.norm(axis)
+    rot = (Rotation.from_rotvec(az * init_up) * Rotation.from_rotvec(el * axis)
+           * Rotation.from_rotvec(roll * view))
+    return rot.apply(rel) + target, rot.apply(init_up)
+
+
+def test_default_front_view_kat():
+    pos, up = get_camera_pos(target=np.array([0.0, 0.0, 0.0]), azimuth=0.0, elevation=0.0,
+                             roll=0.0, distance=3.0)
+    assert np.allclose(pos, [0, 0, 3]) and np.allclose(up, [0, 1, 0])
+
+
+def test_custom_initial_vectors_kat():
+    init_up = np.array([0.0, 0.0, 1.0])
+    pos, up = get_camera_pos(np.zeros(3), 0.0, 0.0, 0.0, 2.0, init_pos=np.array([1.0, 0.0, 0.0]),
+                             init_up=init_up)
+    assert np.linalg.norm(pos) == pytest.approx(2.0)
+    assert np.allclose(up, init_up)
+
+
+@pytest.mark.parametrize("az,el", [(np.pi / 4, np.pi / 6), (1.0, -0.5), (5.5, 1.2), (0.0, 0.0)])
+def test_closed_form_default_camera(az, el):
+    """Camera defaults (init_pos x, init_up z): pos = d(cos el cos az, cos el sin az, sin el),
+    up = (-sin el cos az, -sin el sin az, cos el)."""
+    cam = Camera.from_spherical(np.zeros(3, np.float32), az, el, 0.0, 3.0)
+    pos, up = cam.get_camera_vectors()
+    # the angles enter through float32 rotation vectors (python float * float32 array)
+    a, e = float(np.float32(az)), float(np.float32(el))
+    exp_pos = 3 * np.array([np.cos(e) * np.cos(a), np.cos(e) * np.sin(a), np.sin(e)])
+    exp_up = np.array([-np.sin(e) * np.cos(a), -np.sin(e) * np.sin(a), np.cos(e)])
+    assert np.allclose(pos, exp_pos, atol=1e-12)
+    assert np.allclose(up, exp_up, atol=1e-12)
+
+
+def test_matches_scipy_rotation():
+    rng = np.random.default_rng(0)
+    for _ in range(50):
+        target = rng.normal(size=3).astype(np.float32)
+        init_pos = target + rng.normal(size=3).astype(np.float32)
+        init_up = rng.normal(size=3).astype(np.float32)
+        az, el, roll = (float(v) for v in rng.uniform(-6, 6, 3))
+        dist = float(rng.uniform(0.5, 10))
+        p1, u1 = get_camera_pos(target, az, el, roll, dist, init_pos, init_up)
+        p2, u2 = scipy_camera_pos(target, az, el, roll, dist, init_pos, init_up)
+        assert np.allclose(p1, p2, atol=1e-12, rtol=1e-12)
+        assert np.allclose(u1, u2, atol=1e-12, rtol=1e-12)
+
+
+def test_quaternion_helpers_match_scipy():
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        a, b = rng.normal(size=3) * 2, rng.normal(size=3) * 2
+        x = rng.normal(size=3)
+        q = _quat_mul(_quat_from_rotvec(a), _quat_from_rotvec(b))
+        ref = (Rotation.from_rotvec(a) * Rotation.from_rotvec(b)).apply(x)
+        assert np.allclose(_quat_rotate(q, x), ref, atol=1e-13)
+    small = np.array([1e-5, -2e-5, 3e-6])  # Taylor branch near the identity
+    assert np.allclose(_quat_rotate(_quat_from_rotvec(small), x), Rotation.from_rotvec(small).apply(x), atol=1e-15)
+
+
+def test_invariants():
+    cam = Camera.from_spherical(np.array([0, 0, 0]), np.pi / 4, np.pi / 6, 0.0, 3.0)
+    pos, up = cam.get_camera_vectors()
+    assert np.linalg.norm(pos - cam.target) == pytest.approx(3.0)
+    assert np.linalg.norm(up) == pytest.approx(1.0)
+    params = Camera(target=np.array([1.0, 2.0, 3.0]), azimuth=np.pi / 4, elevation=np.pi / 6, distance=2.0)
+    p, _ = get_camera_pos_from_params(params)
+    assert np.linalg.norm(p - params.target) == pytest.approx(2.0)
+
+
+def test_view_matrix_layout():
+    """Pre-transposed look-at: its transpose maps world to eye space, sending the
+    camera position to the origin and the target to -z."""
+    cam = Camera.from_spherical(np.zeros(3, np.float32), 0.7, 0.3, 0.2, 4.0)
+    V = cam.get_view_matrix()
+    assert V.dtype == np.float32 and V.shape == (4, 4)
+    pos, _ = cam.get_camera_vectors()
+    Vgl = V.astype(np.float64).T
+    eye_pos = Vgl @ np.append(pos, 1.0)
+    assert np.allclose(eye_pos[:3], 0, atol=1e-5)
+    eye_tgt = Vgl @ np.array([0, 0, 0, 1.0])
+    assert np.allclose(eye_tgt[:3], [0, 0, -4.0], atol=1e-5)
+    R = Vgl[:3, :3]
+    assert np.allclose(R @ R.T, np.eye(3), atol=1e-6)
+
+
+def test_projection_matrix_structure():
+    cam = Camera.front_view(distance=3.0)
+    P = cam.get_projection_matrix(16 / 9)
+    assert P.dtype == np.float32 and P[3, 3] == 0.0 and P[3, 2] == -1.0
+    f = 1 / np.tan(np.pi / 8)
+    assert P[1, 1] == pytest.approx(f) and P[0, 0] == pytest.approx(f / (16 / 9))
+    assert P[2, 2] == pytest.approx((100.1) / (0.1 - 100)) and P[2, 3] == pytest.approx(2 * 100 * 0.1 / (0.1 - 100))
+    assert Camera(fov=np.pi / 6).get_projection_matrix(1.0)[1, 1] > Camera(fov=np.pi / 2).get_projection_matrix(1.0)[1, 1]
+
+
+@pytest.mark.parametrize("kw,match", [
+    ({"distance": -1.0}, "distance"), ({"fov": 0.0}, "fov"), ({"fov": np.pi}, "fov"),
+    ({"near_plane": 0.0}, "near_plane"), ({"far_plane": 0.05}, "far_plane"),
+    ({"init_up": np.zeros(3)}, "init_up"), ({"target": np.zeros(2)}, "target"),
+    ({"azimuth": "a"}, "azimuth"),
+])
+def test_camera_validation(kw, match):
+    with pytest.raises(ValueError, match=match):
+        Camera(**kw)
+
+
+def test_get_camera_pos_validation():
+    t = np.zeros(3)
+    with pytest.raises(ValueError, match="target must be a 3D numpy array"):
+        get_camera_pos([0, 0, 0], 0.0, 0.0, 0.0, 3.0)
+    with pytest.raises(ValueError, match="distance must be positive"):
+        get_camera_pos(t, 0.0, 0.0, 0.0, -1.0)
+    with pytest.raises(ValueError, match="init_pos must be a 3D numpy array"):
+        get_camera_pos(t, 0.0, 0.0, 0.0, 3.0, init_pos=[1, 0, 0])
+    with pytest.raises(ValueError, match="init_up must be a 3D numpy array"):
+        get_camera_pos(t, 0.0, 0.0, 0.0, 3.0, init_up=[0, 1, 0])
+    with pytest.raises(CameraError):
+        validate_camera_angles(float("nan"), 0.0, 0.0)
+    with pytest.raises(CameraError):
+        validate_camera_angles(np.float32(1.0), 0.0, 0.0)
+
+
+def test_presets_and_serialisation(tmp_path):
+    iso = Camera.isometric_view(distance=5.0)
+    assert (iso.azimuth, iso.elevation, iso.distance) == (np.pi / 4, np.pi / 6, 5.0)
+    assert Camera.side_view().azimuth == np.pi / 2 and Camera.top_view().elevation == np.pi / 2
+    d = iso.to_dict()
+    json.dumps(d)
+    assert Camera.from_dict(d).to_dict() == d
+    f = tmp_path / "cam.json"
+    iso.save_to_file(f)
+    assert Camera.load_from_file(f).to_dict() == d
+    assert iso.copy().to_dict() == d
+    assert "azimuth" in repr(iso)
+    assert degrees_to_radians(a=180.0)["a"] == pytest.approx(np.pi)
+    assert radians_to_degrees(a=np.pi)["a"] == pytest.approx(180.0)
