@@ -39,10 +39,11 @@ static int fail_hip(hipError_t e, const char *where)
 struct hvr_volume {
     hvr_ctx *ctx;
     int s0, s1, s2;
-    int fmt;         // TexFmt
-    int lb;          // log2 brick edge
-    int nbx, nby, nbz;
-    uint64_t ntexels;
+    int fmt;               // TexFmt
+    int lb;                // log2 of the brick interior edge
+    uint32_t nbx, nby, nbz;
+    uint32_t brick_bytes;  // (2^lb + 1)^3 texels
+    uint64_t bytes;
     void *d_texels;
     bool scalar_loaded, normals_loaded;
 };
@@ -72,7 +73,6 @@ struct hvr_ctx {
     int timing_max, timing_n;
 };
 
-static const int kLutLdsMax = 4096;  // 64 KiB of LDS per block
 
 // ---------------------------------------------------------------------------
 // FP64 uniform preparation
@@ -140,7 +140,8 @@ static FrameConst frame_const(const hvr_ctx *c)
         f.ext[k] = p.bmax[k] - p.bmin[k];  // float subtraction, as in the shader
     }
     f.step = p.step;
-    f.alpha_k = (float)((double)p.step / (double)p.ref_step);
+    // exp(-a*step/ref) = exp2(-a * log2(e) * step/ref)
+    f.alpha_k = (float)(1.4426950408889634 * (double)p.step / (double)p.ref_step);
     f.ka = p.ambient;
     f.kd = p.diffuse;
     f.tau = p.et_threshold;
@@ -157,7 +158,10 @@ static FrameConst frame_const(const hvr_ctx *c)
     return f;
 }
 
-static TexDesc tex_desc(const hvr_volume *v, int role = HVR_ROLE_SCALAR)
+// Texture descriptor for a frame: texture x samples world z, y samples y,
+// z samples x (the shader's swizzle), so k0 = s0/ext_z, k1 = s1/ext_y,
+// k2 = s2/ext_x with ext the float box extent of the in-box test.
+static TexDesc tex_desc(const hvr_volume *v, int role, const FrameConst &fc)
 {
     TexDesc t;
     memset(&t, 0, sizeof(t));
@@ -166,13 +170,13 @@ static TexDesc tex_desc(const hvr_volume *v, int role = HVR_ROLE_SCALAR)
     t.s0 = v->s0;
     t.s1 = v->s1;
     t.s2 = v->s2;
-    t.fs0 = (float)v->s0;
-    t.fs1 = (float)v->s1;
-    t.fs2 = (float)v->s2;
+    t.k0 = (float)((double)v->s0 / (double)fc.ext[2]);
+    t.k1 = (float)((double)v->s1 / (double)fc.ext[1]);
+    t.k2 = (float)((double)v->s2 / (double)fc.ext[0]);
     t.lb = v->lb;
-    const uint32_t b3 = 1u << (3 * v->lb);
-    t.bsy = (uint32_t)v->nbx * b3;
-    t.bsz = (uint32_t)v->nbx * (uint32_t)v->nby * b3;
+    t.nbx = v->nbx;
+    t.nby = v->nby;
+    t.brick_bytes = v->brick_bytes;
     t.fmt = v->fmt;
     t.role = role;
     return t;
@@ -180,14 +184,23 @@ static TexDesc tex_desc(const hvr_volume *v, int role = HVR_ROLE_SCALAR)
 
 // ---------------------------------------------------------------------------
 // kernel dispatch
-template <int SF, int NM, bool COUNT>
+template <int SF, int NM, int LB, bool COUNT>
 static void launch_march_t(hvr_ctx *c, const FrameConst &fc, const TexDesc &ts, const TexDesc &tn,
                            int ntx, int nblocks)
 {
-    const size_t lds = NM == NM_ANY ? 0 : (size_t)c->lut_n * sizeof(float4);
-    hipLaunchKernelGGL((k_march<SF, NM, COUNT>), dim3(nblocks), dim3(256), lds, c->stream, fc, ts, tn,
+    const size_t lds = NM == NM_ANY ? 0 : 2 * (size_t)c->lut_n * sizeof(float4);
+    hipLaunchKernelGGL((k_march<SF, NM, LB, COUNT>), dim3(nblocks), dim3(256), lds, c->stream, fc, ts, tn,
                        (const float4 *)c->d_lut, c->lut_n, (const RayRec *)c->d_rays, c->d_out,
                        c->d_counts, ntx, nblocks);
+}
+
+template <int SF, int LB, bool COUNT>
+static void launch_fast(hvr_ctx *c, const FrameConst &fc, const TexDesc &ts, int nm, int ntx, int nblocks)
+{
+    const TexDesc none = tex_desc(nullptr, 0, fc);
+    if (nm == NM_ZERO) launch_march_t<SF, NM_ZERO, LB, COUNT>(c, fc, ts, none, ntx, nblocks);
+    else if (nm == NM_ALIAS) launch_march_t<SF, NM_ALIAS, LB, COUNT>(c, fc, ts, none, ntx, nblocks);
+    else if constexpr (fmt_has_normal(SF)) launch_march_t<SF, NM_SAME, LB, COUNT>(c, fc, ts, none, ntx, nblocks);
 }
 
 template <bool COUNT>
@@ -196,27 +209,25 @@ static void launch_march(hvr_ctx *c, const FrameConst &fc)
     const int ntx = (c->W + 15) / 16, nty = (c->H + 15) / 16;
     const int nblocks = ntx * nty;
     const hvr_volume *S = c->scalar_vol, *N = c->normal_vol;
-    const TexDesc ts = tex_desc(S, c->scalar_role), tn = tex_desc(N, c->normal_role);
-    const TexDesc none = tex_desc(nullptr);
+    const TexDesc ts = tex_desc(S, c->scalar_role, fc), tn = tex_desc(N, c->normal_role, fc);
+    const TexDesc none = tex_desc(nullptr, 0, fc);
     const bool small_lut = c->lut_n <= kLutLdsMax;
     if (small_lut && !S && !N) {
-        launch_march_t<TF_NONE, NM_ZERO, COUNT>(c, fc, none, none, ntx, nblocks);
+        launch_march_t<TF_NONE, NM_ZERO, 3, COUNT>(c, fc, none, none, ntx, nblocks);
         return;
     }
     // Fast paths: scalar field of one volume, normals from the same texels,
-    // (rho, 0, 0) of the same volume, or unbound.
-    if (small_lut && S && c->scalar_role == HVR_ROLE_SCALAR) {
+    // (rho, 0, 0) of the same volume, or unbound; brick edge 8 or 16.
+    if (small_lut && S && c->scalar_role == HVR_ROLE_SCALAR && (S->lb == 3 || S->lb == 4)) {
         int nm = -1;
         if (!N) nm = NM_ZERO;
         else if (N == S && c->normal_role == HVR_ROLE_SCALAR) nm = NM_ALIAS;
         else if (N == S && c->normal_role == HVR_ROLE_NORMAL && fmt_has_scalar(S->fmt)) nm = NM_SAME;
-#define HVR_CASE(SF)                                                                        \
-    case SF:                                                                                \
-        if (nm == NM_ZERO) launch_march_t<SF, NM_ZERO, COUNT>(c, fc, ts, none, ntx, nblocks);   \
-        else if (nm == NM_ALIAS) launch_march_t<SF, NM_ALIAS, COUNT>(c, fc, ts, none, ntx, nblocks); \
-        else launch_march_t<SF, HVR_NM_SAME_OR_ALIAS(SF), COUNT>(c, fc, ts, none, ntx, nblocks);  \
+#define HVR_CASE(SF)                                                                          \
+    case SF:                                                                                  \
+        if (S->lb == 3) launch_fast<SF, 3, COUNT>(c, fc, ts, nm, ntx, nblocks);               \
+        else launch_fast<SF, 4, COUNT>(c, fc, ts, nm, ntx, nblocks);                          \
         return;
-#define HVR_NM_SAME_OR_ALIAS(SF) (fmt_has_normal(SF) ? NM_SAME : NM_ALIAS)
         if (nm >= 0) {
             switch (S->fmt) {
                 HVR_CASE(TF_S_F32)
@@ -232,9 +243,8 @@ static void launch_march(hvr_ctx *c, const FrameConst &fc)
             }
         }
 #undef HVR_CASE
-#undef HVR_NM_SAME_OR_ALIAS
     }
-    launch_march_t<TF_NONE, NM_ANY, COUNT>(c, fc, ts, tn, ntx, nblocks);
+    launch_march_t<TF_NONE, NM_ANY, 3, COUNT>(c, fc, ts, tn, ntx, nblocks);
 }
 
 static int enqueue_frame(hvr_ctx *c, bool count, uchar4 *dst)
@@ -295,7 +305,7 @@ int hvr_create(int device, int width, int height, hvr_ctx **out)
     c->device = device;
     c->W = width;
     c->H = height;
-    c->brick_log2 = 2;
+    c->brick_log2 = 3;
     const size_t npix = (size_t)width * height;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&c->d_out, npix * sizeof(uchar4));
@@ -344,7 +354,7 @@ int hvr_destroy(hvr_ctx *c)
 int hvr_set_brick_log2(hvr_ctx *c, int lb)
 {
     if (!c) return fail(HVR_EINVAL, "ctx is null");
-    if (lb < 0 || lb > 4) return fail(HVR_EINVAL, "brick log2 must be in [0, 4]");
+    if (lb < 2 || lb > 4) return fail(HVR_EINVAL, "brick log2 must be in [2, 4]");
     c->brick_log2 = lb;
     return HVR_OK;
 }
@@ -380,8 +390,10 @@ int hvr_volume_create(hvr_ctx *c, int s0, int s1, int s2, int sf, int nf, hvr_vo
     if (s0 <= 0 || s1 <= 0 || s2 <= 0) return fail(HVR_EINVAL, "volume dimensions must be positive");
     const int fmt = texel_format(sf, nf);
     if (fmt < 0) return fail(HVR_EINVAL, "unsupported scalar/normal format combination");
+    if (s0 > (1 << 20) || s1 > (1 << 20) || s2 > (1 << 20))
+        return fail(HVR_EINVAL, "volume dimension above 2^20");
     const int lb = c->brick_log2;
-    const int B = 1 << lb;
+    const uint64_t E = (1u << lb) + 1u;
     hvr_volume *v = new hvr_volume();
     memset(v, 0, sizeof(*v));
     v->ctx = c;
@@ -390,15 +402,17 @@ int hvr_volume_create(hvr_ctx *c, int s0, int s1, int s2, int sf, int nf, hvr_vo
     v->s2 = s2;
     v->fmt = fmt;
     v->lb = lb;
-    v->nbx = (s0 + B - 1) / B;
-    v->nby = (s1 + B - 1) / B;
-    v->nbz = (s2 + B - 1) / B;
-    v->ntexels = (uint64_t)v->nbx * v->nby * v->nbz << (3 * lb);
-    if (v->ntexels >= (1ull << 32)) {
+    v->nbx = bricks_along(s0, lb);
+    v->nby = bricks_along(s1, lb);
+    v->nbz = bricks_along(s2, lb);
+    v->brick_bytes = (uint32_t)(E * E * E * (uint64_t)texel_bytes(fmt));
+    const uint64_t nbricks = (uint64_t)v->nbx * v->nby * v->nbz;
+    if (nbricks >= (1ull << 32)) {
         delete v;
-        return fail(HVR_EINVAL, "volume too large (texel index exceeds 32 bits)");
+        return fail(HVR_EINVAL, "volume too large (brick index exceeds 32 bits)");
     }
-    const size_t bytes = (size_t)v->ntexels * texel_bytes(fmt);
+    v->bytes = nbricks * v->brick_bytes;
+    const size_t bytes = (size_t)v->bytes;
     hipError_t e = hipSetDevice(c->device);
     if (e == hipSuccess) e = hipMalloc(&v->d_texels, bytes);
     if (e == hipSuccess) e = hipMemsetAsync(v->d_texels, 0, bytes, c->stream);
@@ -432,15 +446,34 @@ static int upload_part(hvr_volume *v, const float *data, int part)
         HIPCHK(hipMalloc(&c->d_stage, need));
         c->stage_bytes = need;
     }
-    const TexDesc t = tex_desc(v);
     for (uint64_t z0 = 0; z0 < (uint64_t)v->s2; z0 += zper) {
         const uint64_t nz = (z0 + zper <= (uint64_t)v->s2) ? zper : v->s2 - z0;
         const uint64_t nvox = nz * plane;
         HIPCHK(hipMemcpyAsync(c->d_stage, data + z0 * plane * comps, nvox * comps * sizeof(float),
                               hipMemcpyHostToDevice, c->stream));
         const unsigned nblk = (unsigned)((nvox + 255) / 256);
-        hipLaunchKernelGGL(k_upload, dim3(nblk), dim3(256), 0, c->stream, v->d_texels, v->fmt, v->s0,
-                           v->s1, v->lb, t.bsy, t.bsz, (const float *)c->d_stage, part, (int)z0, nvox);
+#define HVR_UP(F)                                                                                      \
+    case F:                                                                                            \
+        hipLaunchKernelGGL(k_upload<F>, dim3(nblk), dim3(256), 0, c->stream, (char *)v->d_texels, v->s0, \
+                           v->s1, v->s2, v->lb, v->nbx, v->nby, v->nbz, v->brick_bytes,                     \
+                           (const float *)c->d_stage,                                                   \
+                           part, (int)z0, nvox);                                                        \
+        break;
+        switch (v->fmt) {
+            HVR_UP(TF_S_F32)
+            HVR_UP(TF_S_F16)
+            HVR_UP(TF_S_U8)
+            HVR_UP(TF_N_F32)
+            HVR_UP(TF_N_SN8)
+            HVR_UP(TF_C_F32_F32)
+            HVR_UP(TF_C_F16_F32)
+            HVR_UP(TF_C_U8_F32)
+            HVR_UP(TF_C_F32_SN8)
+            HVR_UP(TF_C_F16_SN8)
+            HVR_UP(TF_C_U8_SN8)
+        default: return fail(HVR_EINVAL, "bad texel format");
+        }
+#undef HVR_UP
         HIPCHK(hipGetLastError());
         // the staging buffer is reused by the next chunk
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -469,7 +502,7 @@ int hvr_volume_upload_normals(hvr_volume *v, const float *data)
 int hvr_volume_bytes(const hvr_volume *v, uint64_t *bytes)
 {
     if (!v || !bytes) return fail(HVR_EINVAL, "null argument");
-    *bytes = v->ntexels * (uint64_t)texel_bytes(v->fmt);
+    *bytes = v->bytes;
     return HVR_OK;
 }
 

@@ -1,5 +1,6 @@
 // hvr_device.h -- device code of libhipvr: texel formats, GL-style sampling,
-// the ray-setup kernel and the march-and-composite kernel (gfx950, wave64).
+// the ray-setup kernel, the march-and-composite kernel and the upload kernel
+// (gfx950, wave64).
 //
 // Semantics follow the reference fragment shader pyvr/shaders/volume.frag.glsl
 // and the GL state of pyvr/moderngl_renderer/manager.py (paths relative to the
@@ -20,8 +21,9 @@ namespace hvr {
 // Decoding yields (nx, ny, nz, rho).  For scalar-only texels the normal part
 // reads (rho, 0, 0), which is what a 3-channel GL sampler returns from a
 // 1-channel texture; for normal-only texels rho reads nx (the .r channel).
-// SNORM8 normals are kept as raw integers in the sampling fast paths: the
-// normal is normalised afterwards, so the 1/127 scale cancels.
+// In the fast paths SNORM8 normals stay raw integers (the normal is
+// normalised afterwards, so the 1/127 scale cancels) and UNORM8 scalars are
+// scaled by 1/255 after interpolation (filtering is linear).
 enum TexFmt : int {
     TF_NONE = 0,
     TF_S_F32 = 1,      // float                                    4 B
@@ -62,76 +64,69 @@ __host__ __device__ constexpr bool fmt_u8_scalar(int f)
     return f == TF_S_U8 || f == TF_C_U8_F32 || f == TF_C_U8_SN8;
 }
 
-// Layout of a device texture: GL size (s0, s1, s2) (x fastest), stored in
-// cubic bricks of edge 2^lb (lb = 0: plain linear layout).  Texel (x, y, z)
-// lives at offx(x) + offy(y) + offz(z), see brick_off_*.
+// ---------------------------------------------------------------------------
+// Brick layout with apron.  A texture of GL size (s0, s1, s2) is addressed in
+// extended coordinates X = x + 1 in [0, s+1] per axis, where X = 0 and
+// X = s+1 hold copies of the edge texels (x = 0 and x = s-1): sampling the
+// extended texture without clamping is exactly GL's CLAMP_TO_EDGE.  The
+// extended texture is cut into bricks of B = 2^lb interior texels per axis,
+// each stored with one more texel per axis (its neighbour's first), so a
+// brick holds E^3 texels, E = B + 1, x fastest.  A trilinear footprint with
+// lower corner X0 lies in brick X0 >> lb at local X0 & (B-1) .. +1: one base
+// address per sample, the eight taps at fixed offsets.
 struct TexDesc {
     const void *base;
-    int s0, s1, s2;
-    float fs0, fs1, fs2;
-    int lb;
-    uint32_t bsy, bsz;  // texels per brick row (nbx*B^3) and per brick slab
+    int s0, s1, s2;        // GL size
+    float k0, k1, k2;      // s_axis / world extent of the axis it samples
+    int lb;                // log2 of the brick interior edge
+    uint32_t nbx, nby;     // bricks along x and y
+    uint32_t brick_bytes;  // E^3 * texel bytes
     int fmt;
-    int role;           // field a sampler reads: 0 scalar, 1 normal
+    int role;              // field a sampler reads: 0 scalar, 1 normal
 };
 
-__device__ __forceinline__ uint32_t brick_off_x(int i, int lb)
-{
-    const int m = (1 << lb) - 1;
-    return ((uint32_t)(i >> lb) << (3 * lb)) + (uint32_t)(i & m);
-}
-__device__ __forceinline__ uint32_t brick_off_y(int i, int lb, uint32_t bsy)
-{
-    const int m = (1 << lb) - 1;
-    return (uint32_t)(i >> lb) * bsy + ((uint32_t)(i & m) << lb);
-}
-__device__ __forceinline__ uint32_t brick_off_z(int i, int lb, uint32_t bsz)
-{
-    const int m = (1 << lb) - 1;
-    return (uint32_t)(i >> lb) * bsz + ((uint32_t)(i & m) << (2 * lb));
-}
+__host__ __device__ constexpr uint32_t bricks_along(int s, int lb) { return ((uint32_t)s >> lb) + 1u; }
 
 __device__ __forceinline__ float sn8f(uint32_t w, int sh) { return (float)(int8_t)((w >> sh) & 0xffu); }
 
-// Decode one texel to (nx, ny, nz, rho); SNORM8 normals as raw integers,
-// UNORM8 scalars as raw integers (0..255).
+// Decode the texel at byte address p to (nx, ny, nz, rho) (raw SNORM8/UNORM8).
 template <int F>
-__device__ __forceinline__ float4 fetch4(const void *base, uint32_t idx)
+__device__ __forceinline__ float4 fetch4(const char *p)
 {
     if constexpr (F == TF_S_F32) {
-        float v = ((const float *)base)[idx];
+        const float v = *(const float *)p;
         return make_float4(v, 0.f, 0.f, v);
     } else if constexpr (F == TF_S_F16) {
-        float v = __half2float(((const __half *)base)[idx]);
+        const float v = __half2float(*(const __half *)p);
         return make_float4(v, 0.f, 0.f, v);
     } else if constexpr (F == TF_S_U8) {
-        float v = (float)((const uint8_t *)base)[idx];
+        const float v = (float)*(const uint8_t *)p;
         return make_float4(v, 0.f, 0.f, v);
     } else if constexpr (F == TF_N_F32) {
-        float4 t = ((const float4 *)base)[idx];
+        const float4 t = *(const float4 *)p;
         return make_float4(t.x, t.y, t.z, t.x);
     } else if constexpr (F == TF_N_SN8) {
-        uint32_t w = ((const uint32_t *)base)[idx];
-        float x = sn8f(w, 0);
+        const uint32_t w = *(const uint32_t *)p;
+        const float x = sn8f(w, 0);
         return make_float4(x, sn8f(w, 8), sn8f(w, 16), x);
     } else if constexpr (F == TF_C_F32_F32) {
-        return ((const float4 *)base)[idx];
+        return *(const float4 *)p;
     } else if constexpr (F == TF_C_F16_F32) {
-        float4 t = ((const float4 *)base)[idx];
-        __half h = __ushort_as_half((unsigned short)(__float_as_uint(t.w) & 0xffffu));
+        const float4 t = *(const float4 *)p;
+        const __half h = __ushort_as_half((unsigned short)(__float_as_uint(t.w) & 0xffffu));
         return make_float4(t.x, t.y, t.z, __half2float(h));
     } else if constexpr (F == TF_C_U8_F32) {
-        float4 t = ((const float4 *)base)[idx];
+        const float4 t = *(const float4 *)p;
         return make_float4(t.x, t.y, t.z, (float)(__float_as_uint(t.w) & 0xffu));
     } else if constexpr (F == TF_C_F32_SN8) {
-        uint2 t = ((const uint2 *)base)[idx];
+        const uint2 t = *(const uint2 *)p;
         return make_float4(sn8f(t.y, 0), sn8f(t.y, 8), sn8f(t.y, 16), __uint_as_float(t.x));
     } else if constexpr (F == TF_C_F16_SN8) {
-        uint2 t = ((const uint2 *)base)[idx];
-        __half h = __ushort_as_half((unsigned short)(t.x & 0xffffu));
+        const uint2 t = *(const uint2 *)p;
+        const __half h = __ushort_as_half((unsigned short)(t.x & 0xffffu));
         return make_float4(sn8f(t.y, 0), sn8f(t.y, 8), sn8f(t.y, 16), __half2float(h));
     } else if constexpr (F == TF_C_U8_SN8) {
-        uint32_t w = ((const uint32_t *)base)[idx];
+        const uint32_t w = *(const uint32_t *)p;
         return make_float4(sn8f(w, 0), sn8f(w, 8), sn8f(w, 16), (float)(w >> 24));
     } else {
         return make_float4(0.f, 0.f, 0.f, 0.f);
@@ -140,30 +135,30 @@ __device__ __forceinline__ float4 fetch4(const void *base, uint32_t idx)
 
 // Scalar-only decode (avoids interpolating unused channels).
 template <int F>
-__device__ __forceinline__ float fetch1(const void *base, uint32_t idx)
+__device__ __forceinline__ float fetch1(const char *p)
 {
-    if constexpr (F == TF_S_F32) return ((const float *)base)[idx];
-    else if constexpr (F == TF_S_F16) return __half2float(((const __half *)base)[idx]);
-    else if constexpr (F == TF_S_U8) return (float)((const uint8_t *)base)[idx];
-    else return fetch4<F>(base, idx).w;
+    if constexpr (F == TF_S_F32) return *(const float *)p;
+    else if constexpr (F == TF_S_F16) return __half2float(*(const __half *)p);
+    else if constexpr (F == TF_S_U8) return (float)*(const uint8_t *)p;
+    else return fetch4<F>(p).w;
 }
 
 // Runtime-format decode with exact GL scaling (generic path).
-__device__ __forceinline__ float4 fetch_any(int fmt, const void *base, uint32_t idx)
+__device__ __forceinline__ float4 fetch_any(int fmt, const char *p)
 {
     float4 t;
     switch (fmt) {
-    case TF_S_F32: t = fetch4<TF_S_F32>(base, idx); break;
-    case TF_S_F16: t = fetch4<TF_S_F16>(base, idx); break;
-    case TF_S_U8: t = fetch4<TF_S_U8>(base, idx); break;
-    case TF_N_F32: t = fetch4<TF_N_F32>(base, idx); break;
-    case TF_N_SN8: t = fetch4<TF_N_SN8>(base, idx); break;
-    case TF_C_F32_F32: t = fetch4<TF_C_F32_F32>(base, idx); break;
-    case TF_C_F16_F32: t = fetch4<TF_C_F16_F32>(base, idx); break;
-    case TF_C_U8_F32: t = fetch4<TF_C_U8_F32>(base, idx); break;
-    case TF_C_F32_SN8: t = fetch4<TF_C_F32_SN8>(base, idx); break;
-    case TF_C_F16_SN8: t = fetch4<TF_C_F16_SN8>(base, idx); break;
-    case TF_C_U8_SN8: t = fetch4<TF_C_U8_SN8>(base, idx); break;
+    case TF_S_F32: t = fetch4<TF_S_F32>(p); break;
+    case TF_S_F16: t = fetch4<TF_S_F16>(p); break;
+    case TF_S_U8: t = fetch4<TF_S_U8>(p); break;
+    case TF_N_F32: t = fetch4<TF_N_F32>(p); break;
+    case TF_N_SN8: t = fetch4<TF_N_SN8>(p); break;
+    case TF_C_F32_F32: t = fetch4<TF_C_F32_F32>(p); break;
+    case TF_C_F16_F32: t = fetch4<TF_C_F16_F32>(p); break;
+    case TF_C_U8_F32: t = fetch4<TF_C_U8_F32>(p); break;
+    case TF_C_F32_SN8: t = fetch4<TF_C_F32_SN8>(p); break;
+    case TF_C_F16_SN8: t = fetch4<TF_C_F16_SN8>(p); break;
+    case TF_C_U8_SN8: t = fetch4<TF_C_U8_SN8>(p); break;
     default: return make_float4(0.f, 0.f, 0.f, 0.f);
     }
     const float k127 = 1.0f / 127.0f, k255 = 1.0f / 255.0f;
@@ -182,81 +177,105 @@ __device__ __forceinline__ float4 fetch_any(int fmt, const void *base, uint32_t 
 
 // ---------------------------------------------------------------------------
 // Trilinear sampling (GL_LINEAR, CLAMP_TO_EDGE, no mipmaps;
-// manager.py:95-101, 123-129).  Coordinate (cx, cy, cz) in [0, 1]:
-// u = c*size - 1/2 per axis, taps floor(u) and floor(u)+1 clamped to
-// [0, size-1], weights frac(u).
-struct Taps {
-    uint32_t i000, i100, i010, i110, i001, i101, i011, i111;
-    float a, b, g;
+// manager.py:95-101, 123-129).  Per axis the GL rule is u = c*size - 1/2,
+// taps floor(u), floor(u)+1 (clamped), weight frac(u).  With the extended
+// coordinates of the brick layout the lower tap is X0 = floor(c*size + 1/2)
+// in [0, size] and no clamp is needed.  (w0, w1, w2) is the position in
+// world units from the box minimum along the axes sampled by texture x, y, z
+// (world z, y, x: the shader's swizzle, volume.frag.glsl:90).
+struct Footprint {
+    const char *p;    // byte address of the lower-corner texel
+    float a, b, g;    // weights along texture x, y, z
 };
 
-__device__ __forceinline__ Taps make_taps(const TexDesc &t, float cx, float cy, float cz)
+template <int T, int LB>
+__device__ __forceinline__ Footprint footprint(const TexDesc &t, float w0, float w1, float w2)
 {
-    float u = fmaf(cx, t.fs0, -0.5f), v = fmaf(cy, t.fs1, -0.5f), w = fmaf(cz, t.fs2, -0.5f);
-    float fu = floorf(u), fv = floorf(v), fw = floorf(w);
-    Taps k;
-    k.a = u - fu;
-    k.b = v - fv;
-    k.g = w - fw;
-    int x0 = (int)fu, y0 = (int)fv, z0 = (int)fw;
-    int x1 = min(x0 + 1, t.s0 - 1), y1 = min(y0 + 1, t.s1 - 1), z1 = min(z0 + 1, t.s2 - 1);
-    x0 = max(x0, 0);
-    y0 = max(y0, 0);
-    z0 = max(z0, 0);
-    uint32_t ox0 = brick_off_x(x0, t.lb), ox1 = brick_off_x(x1, t.lb);
-    uint32_t oy0 = brick_off_y(y0, t.lb, t.bsy), oy1 = brick_off_y(y1, t.lb, t.bsy);
-    uint32_t oz0 = brick_off_z(z0, t.lb, t.bsz), oz1 = brick_off_z(z1, t.lb, t.bsz);
-    uint32_t y0z0 = oy0 + oz0, y1z0 = oy1 + oz0, y0z1 = oy0 + oz1, y1z1 = oy1 + oz1;
-    k.i000 = ox0 + y0z0;
-    k.i100 = ox1 + y0z0;
-    k.i010 = ox0 + y1z0;
-    k.i110 = ox1 + y1z0;
-    k.i001 = ox0 + y0z1;
-    k.i101 = ox1 + y0z1;
-    k.i011 = ox0 + y1z1;
-    k.i111 = ox1 + y1z1;
-    return k;
+    const float u = fmaf(w0, t.k0, 0.5f), v = fmaf(w1, t.k1, 0.5f), w = fmaf(w2, t.k2, 0.5f);
+    const float fu = floorf(u), fv = floorf(v), fw = floorf(w);
+    Footprint f;
+    f.a = u - fu;
+    f.b = v - fv;
+    f.g = w - fw;
+    const uint32_t X = (uint32_t)(int)fu, Y = (uint32_t)(int)fv, Z = (uint32_t)(int)fw;
+    constexpr uint32_t M = (1u << LB) - 1u, E = (1u << LB) + 1u;
+    const uint32_t brick = ((Z >> LB) * t.nby + (Y >> LB)) * t.nbx + (X >> LB);
+    const uint32_t local = ((Z & M) * E + (Y & M)) * E + (X & M);
+    f.p = (const char *)t.base + (uint64_t)brick * t.brick_bytes + (uint64_t)(local * (uint32_t)T);
+    return f;
 }
 
-__device__ __forceinline__ float lerpf(float x, float y, float t) { return fmaf(t, y - x, x); }
-__device__ __forceinline__ float4 lerp4(float4 x, float4 y, float t)
+// The eight trilinear weights, x fastest: w[i] for tap (i&1, (i>>1)&1, i>>2).
+__device__ __forceinline__ void tri_weights(float a, float b, float g, float w[8])
 {
-    return make_float4(lerpf(x.x, y.x, t), lerpf(x.y, y.y, t), lerpf(x.z, y.z, t), lerpf(x.w, y.w, t));
+    const float a1 = 1.f - a, b1 = 1.f - b, g1 = 1.f - g;
+    const float w00 = b1 * g1, w10 = b * g1, w01 = b1 * g, w11 = b * g;
+    w[0] = a1 * w00; w[1] = a * w00;
+    w[2] = a1 * w10; w[3] = a * w10;
+    w[4] = a1 * w01; w[5] = a * w01;
+    w[6] = a1 * w11; w[7] = a * w11;
 }
 
-template <int F>
-__device__ __forceinline__ float4 sample4(const TexDesc &t, const Taps &k)
+template <int F, int LB>
+__device__ __forceinline__ float4 sample4(const Footprint &f)
 {
-    float4 c000 = fetch4<F>(t.base, k.i000), c100 = fetch4<F>(t.base, k.i100);
-    float4 c010 = fetch4<F>(t.base, k.i010), c110 = fetch4<F>(t.base, k.i110);
-    float4 c001 = fetch4<F>(t.base, k.i001), c101 = fetch4<F>(t.base, k.i101);
-    float4 c011 = fetch4<F>(t.base, k.i011), c111 = fetch4<F>(t.base, k.i111);
-    float4 e0 = lerp4(c000, c100, k.a), e1 = lerp4(c010, c110, k.a);
-    float4 e2 = lerp4(c001, c101, k.a), e3 = lerp4(c011, c111, k.a);
-    return lerp4(lerp4(e0, e1, k.b), lerp4(e2, e3, k.b), k.g);
+    constexpr int T = texel_bytes(F), E = (1 << LB) + 1;
+    float w[8];
+    tri_weights(f.a, f.b, f.g, w);
+    const float4 t0 = fetch4<F>(f.p), t1 = fetch4<F>(f.p + T);
+    const float4 t2 = fetch4<F>(f.p + E * T), t3 = fetch4<F>(f.p + (E + 1) * T);
+    const float4 t4 = fetch4<F>(f.p + E * E * T), t5 = fetch4<F>(f.p + (E * E + 1) * T);
+    const float4 t6 = fetch4<F>(f.p + (E * E + E) * T), t7 = fetch4<F>(f.p + (E * E + E + 1) * T);
+    float4 r;
+    r.x = w[0] * t0.x; r.y = w[0] * t0.y; r.z = w[0] * t0.z; r.w = w[0] * t0.w;
+#define HVR_ACC(k, tk) r.x = fmaf(w[k], tk.x, r.x); r.y = fmaf(w[k], tk.y, r.y); \
+                       r.z = fmaf(w[k], tk.z, r.z); r.w = fmaf(w[k], tk.w, r.w)
+    HVR_ACC(1, t1); HVR_ACC(2, t2); HVR_ACC(3, t3); HVR_ACC(4, t4);
+    HVR_ACC(5, t5); HVR_ACC(6, t6); HVR_ACC(7, t7);
+#undef HVR_ACC
+    return r;
 }
 
-template <int F>
-__device__ __forceinline__ float sample1(const TexDesc &t, const Taps &k)
+template <int F, int LB>
+__device__ __forceinline__ float sample1(const Footprint &f)
 {
-    float c000 = fetch1<F>(t.base, k.i000), c100 = fetch1<F>(t.base, k.i100);
-    float c010 = fetch1<F>(t.base, k.i010), c110 = fetch1<F>(t.base, k.i110);
-    float c001 = fetch1<F>(t.base, k.i001), c101 = fetch1<F>(t.base, k.i101);
-    float c011 = fetch1<F>(t.base, k.i011), c111 = fetch1<F>(t.base, k.i111);
-    float e0 = lerpf(c000, c100, k.a), e1 = lerpf(c010, c110, k.a);
-    float e2 = lerpf(c001, c101, k.a), e3 = lerpf(c011, c111, k.a);
-    return lerpf(lerpf(e0, e1, k.b), lerpf(e2, e3, k.b), k.g);
+    constexpr int T = texel_bytes(F), E = (1 << LB) + 1;
+    float w[8];
+    tri_weights(f.a, f.b, f.g, w);
+    float r = w[0] * fetch1<F>(f.p);
+    r = fmaf(w[1], fetch1<F>(f.p + T), r);
+    r = fmaf(w[2], fetch1<F>(f.p + E * T), r);
+    r = fmaf(w[3], fetch1<F>(f.p + (E + 1) * T), r);
+    r = fmaf(w[4], fetch1<F>(f.p + E * E * T), r);
+    r = fmaf(w[5], fetch1<F>(f.p + (E * E + 1) * T), r);
+    r = fmaf(w[6], fetch1<F>(f.p + (E * E + E) * T), r);
+    r = fmaf(w[7], fetch1<F>(f.p + (E * E + E + 1) * T), r);
+    return r;
 }
 
-__device__ __forceinline__ float4 sample_any(const TexDesc &t, const Taps &k)
+// Generic path: runtime format and brick size.
+__device__ __forceinline__ float4 sample_any(const TexDesc &t, float w0, float w1, float w2)
 {
-    float4 c000 = fetch_any(t.fmt, t.base, k.i000), c100 = fetch_any(t.fmt, t.base, k.i100);
-    float4 c010 = fetch_any(t.fmt, t.base, k.i010), c110 = fetch_any(t.fmt, t.base, k.i110);
-    float4 c001 = fetch_any(t.fmt, t.base, k.i001), c101 = fetch_any(t.fmt, t.base, k.i101);
-    float4 c011 = fetch_any(t.fmt, t.base, k.i011), c111 = fetch_any(t.fmt, t.base, k.i111);
-    float4 e0 = lerp4(c000, c100, k.a), e1 = lerp4(c010, c110, k.a);
-    float4 e2 = lerp4(c001, c101, k.a), e3 = lerp4(c011, c111, k.a);
-    return lerp4(lerp4(e0, e1, k.b), lerp4(e2, e3, k.b), k.g);
+    const float u = fmaf(w0, t.k0, 0.5f), v = fmaf(w1, t.k1, 0.5f), w = fmaf(w2, t.k2, 0.5f);
+    const float fu = floorf(u), fv = floorf(v), fw = floorf(w);
+    const uint32_t X = (uint32_t)(int)fu, Y = (uint32_t)(int)fv, Z = (uint32_t)(int)fw;
+    const int lb = t.lb;
+    const uint32_t M = (1u << lb) - 1u, E = (1u << lb) + 1u, T = (uint32_t)texel_bytes(t.fmt);
+    const uint32_t brick = ((Z >> lb) * t.nby + (Y >> lb)) * t.nbx + (X >> lb);
+    const uint32_t local = ((Z & M) * E + (Y & M)) * E + (X & M);
+    const char *p = (const char *)t.base + (uint64_t)brick * t.brick_bytes + (uint64_t)local * T;
+    float wt[8];
+    tri_weights(u - fu, v - fv, w - fw, wt);
+    float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t off = ((k & 1) + ((k >> 1) & 1) * E + (k >> 2) * E * E) * T;
+        const float4 c = fetch_any(t.fmt, p + off);
+        r.x = fmaf(wt[k], c.x, r.x);
+        r.y = fmaf(wt[k], c.y, r.y);
+        r.z = fmaf(wt[k], c.z, r.z);
+        r.w = fmaf(wt[k], c.w, r.w);
+    }
+    return r;
 }
 
 // ---------------------------------------------------------------------------
@@ -267,7 +286,7 @@ struct FrameConst {
     float o[3];       // camera_pos
     float bmin[3], bmax[3], ext[3];   // ext = bmax - bmin in float (:26)
     float step;       // step_size
-    float alpha_k;    // step_size / reference_step_size    (:101)
+    float alpha_k;    // log2(e) * step_size / reference_step_size  (:101)
     float ka, kd;     // ambient_light, diffuse_light       (:110)
     float L[3];       // normalize(light_target - light_position) (:107)
     float tau;        // early-termination alpha            (:87)
@@ -369,39 +388,57 @@ __device__ __forceinline__ uchar4 blend_store(const FrameConst &fc, float c0, fl
 // How the normal sampler is served in a march kernel instance.
 enum NormalMode : int {
     NM_SAME = 0,   // normals interleaved in the scalar volume's texels
-    NM_ALIAS = 1,  // scalar-only volume: normal = (rho, 0, 0)
+    NM_ALIAS = 1,  // the scalar volume read as (rho, 0, 0)
     NM_ZERO = 2,   // unbound: normal = 0 -> diffuse term 0
     NM_ANY = 3     // generic: scalar and normal from any volumes/formats
 };
 
 // LUT lookup at (rho, 0.5) with u = rho*N - 1/2, clamp to edge
-// (manager.py:163-181; volume.frag.glsl:96).
-template <typename LutPtr>
-__device__ __forceinline__ float4 lut_sample(LutPtr lut, int n, float fn, float rho)
+// (manager.py:163-181; volume.frag.glsl:96).  `val` holds the N entries and
+// `dlt` the differences to the next entry (0 after the last), so that
+// clamping u to [0, N-1] gives both edge cases and one fma per channel.
+template <typename P>
+__device__ __forceinline__ float4 lut_sample(P val, P dlt, float fn1, float fn, float rho)
 {
-    float u = fminf(fmaxf(fmaf(rho, fn, -0.5f), -1.f), fn);
+    const float u = fminf(fmaxf(fmaf(rho, fn, -0.5f), 0.f), fn1);
     const float fu = floorf(u);
     const float a = u - fu;
-    const int i0 = (int)fu;
-    const int j0 = min(max(i0, 0), n - 1), j1 = min(max(i0 + 1, 0), n - 1);
-    return lerp4(lut[j0], lut[j1], a);
+    const int i = (int)fu;
+    const float4 v = val[i], d = dlt[i];
+    return make_float4(fmaf(a, d.x, v.x), fmaf(a, d.y, v.y), fmaf(a, d.z, v.z), fmaf(a, d.w, v.w));
 }
+
+static constexpr int kLutLdsMax = 2048;  // entries; 2 x 16 B each in LDS
 
 // K2 -- march and composite (volume.frag.glsl:84-121), one thread per ray.
 // A 256-thread block covers 16x16 pixels; each wave64 an 8x8 tile.  Blocks
 // are spread so that each of the 8 XCDs gets a contiguous band of tiles (its
 // own L2 sees a compact part of the volume).  A wave runs while any of its
-// rays is live; a ray stops at max_steps / its iteration bound, when its
-// alpha reaches tau, or when it leaves the box.
-template <int SF, int NM, bool COUNT>
+// rays is live; a ray stops at its iteration bound (max_steps at most), when
+// its alpha reaches tau, or when it leaves the box.
+//
+// In-box test (:88-92): fl((p - bmin)/ext) lies in [0, 1] exactly when
+// fl(p - bmin) lies in [0, ext] (for ext > 0 the quotient of a larger
+// difference rounds above 1), so the test needs no division.
+template <int SF, int NM, int LB, bool COUNT>
 __global__ void __launch_bounds__(256)
 k_march(const FrameConst fc, const TexDesc ts, const TexDesc tn, const float4 *__restrict__ lut_g,
         int lut_n, const RayRec *__restrict__ rays, uchar4 *__restrict__ out,
         uint32_t *__restrict__ counts, int ntx, int nblocks)
 {
     extern __shared__ float4 s_lut[];
+    float4 *s_val = s_lut, *s_dlt = s_lut + lut_n;
     if constexpr (NM != NM_ANY) {
-        for (int i = threadIdx.x; i < lut_n; i += 256) s_lut[i] = lut_g[i];
+        for (int i = threadIdx.x; i < lut_n; i += 256) {
+            const float4 v = lut_g[i];
+            float4 d = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (i + 1 < lut_n) {
+                const float4 n = lut_g[i + 1];
+                d = make_float4(n.x - v.x, n.y - v.y, n.z - v.z, n.w - v.w);
+            }
+            s_val[i] = v;
+            s_dlt[i] = d;
+        }
         __syncthreads();
     }
     const int b = blockIdx.x;
@@ -419,29 +456,27 @@ k_march(const FrameConst fc, const TexDesc ts, const TexDesc tn, const float4 *_
     float C0 = 0.f, C1 = 0.f, C2 = 0.f, A = 0.f;
     uint32_t ns = 0;
     bool entered = false;
-    const float fn = (float)lut_n;
-    const float u8k = fmt_u8_scalar(SF) ? 1.0f / 255.0f : 1.0f;
+    const float fn = (float)lut_n, fn1 = (float)(lut_n - 1);
+    constexpr float u8k = fmt_u8_scalar(SF) ? 1.0f / 255.0f : 1.0f;
     for (int i = 0; i < r.nlim; ++i) {
         if (!(A < fc.tau)) break;
-        // A6: normalised coordinate and inclusive [0,1] test on (z, y, x)
-        const float qx = (x - fc.bmin[0]) / fc.ext[0];
-        const float qy = (y - fc.bmin[1]) / fc.ext[1];
-        const float qz = (z - fc.bmin[2]) / fc.ext[2];
-        const bool valid = qz >= 0.f && qz <= 1.f && qy >= 0.f && qy <= 1.f && qx >= 0.f && qx <= 1.f;
+        const float wx = x - fc.bmin[0], wy = y - fc.bmin[1], wz = z - fc.bmin[2];
+        const bool valid = wz >= 0.f && wz <= fc.ext[2] && wy >= 0.f && wy <= fc.ext[1] &&
+                           wx >= 0.f && wx <= fc.ext[0];
         if (valid) {
             entered = true;
             if constexpr (COUNT) ++ns;
-            // texture x <- q.z (size s0), y <- q.y (s1), z <- q.x (s2)
+            // texture x <- world z (size s0), y <- world y (s1), z <- world x (s2)
             float rho, n0, n1, n2;
             if constexpr (NM == NM_ANY) {
                 rho = 0.f;
                 if (ts.base) {
-                    const float4 t = sample_any(ts, make_taps(ts, qz, qy, qx));
+                    const float4 t = sample_any(ts, wz, wy, wx);
                     rho = ts.role == 0 ? t.w : t.x;
                 }
                 n0 = n1 = n2 = 0.f;
                 if (tn.base) {
-                    const float4 t = sample_any(tn, make_taps(tn, qz, qy, qx));
+                    const float4 t = sample_any(tn, wz, wy, wx);
                     if (tn.role == 1) { n0 = t.x; n1 = t.y; n2 = t.z; }
                     else n0 = t.w;
                 }
@@ -449,39 +484,50 @@ k_march(const FrameConst fc, const TexDesc ts, const TexDesc tn, const float4 *_
                 rho = 0.f;
                 n0 = n1 = n2 = 0.f;
             } else {
-                const Taps k = make_taps(ts, qz, qy, qx);
+                constexpr int T = texel_bytes(SF);
+                const Footprint f = footprint<T, LB>(ts, wz, wy, wx);
                 if constexpr (NM == NM_SAME) {
-                    const float4 t = sample4<SF>(ts, k);
+                    const float4 t = sample4<SF, LB>(f);
                     rho = t.w * u8k;
                     n0 = t.x; n1 = t.y; n2 = t.z;
                 } else {
-                    rho = sample1<SF>(ts, k) * u8k;
+                    rho = sample1<SF, LB>(f) * u8k;
                     n0 = NM == NM_ALIAS ? rho : 0.f;
                     n1 = n2 = 0.f;
                 }
             }
             float4 q;
-            if constexpr (NM == NM_ANY) q = lut_sample(lut_g, lut_n, fn, rho);
-            else q = lut_sample(s_lut, lut_n, fn, rho);
-            // Beer-Lambert opacity correction (:101)
-            const float a = 1.f - __expf(-(q.w * fc.alpha_k));
+            if constexpr (NM == NM_ANY) {
+                // global LUT with clamp-to-edge taps
+                const float u = fminf(fmaxf(fmaf(rho, fn, -0.5f), 0.f), fn1);
+                const float fu = floorf(u);
+                const int i0 = (int)fu, i1 = min(i0 + 1, lut_n - 1);
+                const float aa = u - fu;
+                const float4 v0 = lut_g[i0], v1 = lut_g[i1];
+                q = make_float4(fmaf(aa, v1.x - v0.x, v0.x), fmaf(aa, v1.y - v0.y, v0.y),
+                                fmaf(aa, v1.z - v0.z, v0.z), fmaf(aa, v1.w - v0.w, v0.w));
+            } else {
+                q = lut_sample(s_val, s_dlt, fn1, fn, rho);
+            }
+            // Beer-Lambert opacity correction (:101): 1 - exp(-alpha*step/ref)
+            const float a = 1.f - exp2f(-(q.w * fc.alpha_k));
             // Lambert term with the zero-vector rule: a normal of zero (or
             // subnormal) length has no direction and gives no diffuse light.
             float diff = 0.f;
             if constexpr (NM != NM_ZERO) {
                 const float nn = n0 * n0 + n1 * n1 + n2 * n2;
                 if (fc.L_ok && nn >= 1.17549435e-38f) {
-                    const float rs = rsqrtf(nn);
-                    const float dt = (n0 * fc.L[0] + n1 * fc.L[1] + n2 * fc.L[2]) * rs;
+                    const float dt = (n0 * fc.L[0] + n1 * fc.L[1] + n2 * fc.L[2]) * rsqrtf(nn);
                     diff = fmaxf(dt, 0.f);
                 }
             }
             const float lam = fmaf(fc.kd, diff, fc.ka);
             // front-to-back composite (:112-115)
             const float ta = (1.f - A) * a;
-            C0 = fmaf(ta, q.x * lam, C0);
-            C1 = fmaf(ta, q.y * lam, C1);
-            C2 = fmaf(ta, q.z * lam, C2);
+            const float tl = ta * lam;
+            C0 = fmaf(tl, q.x, C0);
+            C1 = fmaf(tl, q.y, C1);
+            C2 = fmaf(tl, q.z, C2);
             A = A + ta;
         } else if (entered) {
             break;
@@ -496,9 +542,11 @@ k_march(const FrameConst fc, const TexDesc ts, const TexDesc tn, const float4 *_
 }
 
 // K3 -- upload: convert a slab of host float32 data (numpy C order = GL
-// texel order, x fastest) into the volume's texel format and brick layout.
-// part: 1 = scalar field, 2 = normal field (3 floats per voxel).  The other
-// part of an interleaved texel is preserved.
+// texel order, x fastest) into the volume's texel format and scatter each
+// texel to every slot of the brick layout that holds it (its own slot, the
+// apron copy in the previous brick, and the edge replicas at X = 0 and
+// X = s+1).  part: 1 = scalar field, 2 = normal field (3 floats per voxel).
+// The other part of an interleaved texel is preserved.
 __device__ __forceinline__ uint32_t enc_sn8(float v)
 {
     v = fminf(fmaxf(v, -1.f), 1.f);  // NaN -> -1 .. kept finite
@@ -511,9 +559,71 @@ __device__ __forceinline__ uint32_t enc_u8(float v)
     return (uint32_t)rintf(v * 255.f);
 }
 
+// Slots along one axis that hold texel coordinate c of a texture of size s
+// stored in nb bricks: (brick, local) pairs; returns how many (1..4).  A
+// position whose own brick would be brick nb (X = s+1 with s+1 a multiple
+// of B) is only ever read through the apron of brick nb-1.
+__device__ __forceinline__ int axis_slots(int c, int s, int lb, uint32_t nb, uint32_t br[4], uint32_t lo[4])
+{
+    const uint32_t B = 1u << lb, M = B - 1u;
+    int n = 0;
+    uint32_t X[3];
+    int nx = 0;
+    X[nx++] = (uint32_t)c + 1u;
+    if (c == 0) X[nx++] = 0u;
+    if (c == s - 1) X[nx++] = (uint32_t)s + 1u;
+    for (int k = 0; k < nx; ++k) {
+        if ((X[k] >> lb) < nb) {
+            br[n] = X[k] >> lb;
+            lo[n] = X[k] & M;
+            ++n;
+        }
+        if ((X[k] & M) == 0u && X[k] > 0u) {  // apron copy in the previous brick
+            br[n] = (X[k] >> lb) - 1u;
+            lo[n] = B;
+            ++n;
+        }
+    }
+    return n;
+}
+
+template <int F>
+__device__ __forceinline__ void store_part(char *p, int part, const float *src, uint64_t l)
+{
+    if (part == 1) {
+        const float v = src[l];
+        if constexpr (F == TF_S_F32) *(float *)p = v;
+        else if constexpr (F == TF_S_F16) *(__half *)p = __float2half_rn(v);
+        else if constexpr (F == TF_S_U8) *(uint8_t *)p = (uint8_t)enc_u8(v);
+        else if constexpr (F == TF_C_F32_F32) ((float *)p)[3] = v;
+        else if constexpr (F == TF_C_F16_F32) ((uint32_t *)p)[3] = (uint32_t)__half_as_ushort(__float2half_rn(v));
+        else if constexpr (F == TF_C_U8_F32) ((uint32_t *)p)[3] = enc_u8(v);
+        else if constexpr (F == TF_C_F32_SN8) ((float *)p)[0] = v;
+        else if constexpr (F == TF_C_F16_SN8) ((uint32_t *)p)[0] = (uint32_t)__half_as_ushort(__float2half_rn(v));
+        else if constexpr (F == TF_C_U8_SN8) ((uint8_t *)p)[3] = (uint8_t)enc_u8(v);
+    } else {
+        const float nx = src[3 * l], ny = src[3 * l + 1], nz = src[3 * l + 2];
+        const uint32_t packed = enc_sn8(nx) | (enc_sn8(ny) << 8) | (enc_sn8(nz) << 16);
+        if constexpr (F == TF_N_F32 || F == TF_C_F32_F32 || F == TF_C_F16_F32 || F == TF_C_U8_F32) {
+            float *q = (float *)p;
+            q[0] = nx; q[1] = ny; q[2] = nz;
+            if constexpr (F == TF_N_F32) q[3] = 0.f;
+        } else if constexpr (F == TF_N_SN8) {
+            *(uint32_t *)p = packed;
+        } else if constexpr (F == TF_C_F32_SN8 || F == TF_C_F16_SN8) {
+            ((uint32_t *)p)[1] = packed;
+        } else if constexpr (F == TF_C_U8_SN8) {
+            uint8_t *q = (uint8_t *)p;
+            q[0] = (uint8_t)(packed & 0xffu); q[1] = (uint8_t)((packed >> 8) & 0xffu);
+            q[2] = (uint8_t)((packed >> 16) & 0xffu);
+        }
+    }
+}
+
+template <int F>
 __global__ void __launch_bounds__(256)
-k_upload(void *__restrict__ dst, int fmt, int s0, int s1, int lb, uint32_t bsy, uint32_t bsz,
-         const float *__restrict__ src, int part, int z0, uint64_t nvox)
+k_upload(char *__restrict__ dst, int s0, int s1, int s2, int lb, uint32_t nbx, uint32_t nby, uint32_t nbz,
+         uint32_t brick_bytes, const float *__restrict__ src, int part, int z0, uint64_t nvox)
 {
     const uint64_t l = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     if (l >= nvox) return;
@@ -521,52 +631,19 @@ k_upload(void *__restrict__ dst, int fmt, int s0, int s1, int lb, uint32_t bsy, 
     const int z = (int)(l / plane) + z0;
     const uint64_t rem = l % plane;
     const int y = (int)(rem / (uint64_t)s0), x = (int)(rem % (uint64_t)s0);
-    const uint32_t idx = brick_off_x(x, lb) + brick_off_y(y, lb, bsy) + brick_off_z(z, lb, bsz);
-    if (part == 1) {
-        const float v = src[l];
-        switch (fmt) {
-        case TF_S_F32: ((float *)dst)[idx] = v; break;
-        case TF_S_F16: ((__half *)dst)[idx] = __float2half_rn(v); break;
-        case TF_S_U8: ((uint8_t *)dst)[idx] = (uint8_t)enc_u8(v); break;
-        case TF_C_F32_F32: ((float *)dst)[4 * (uint64_t)idx + 3] = v; break;
-        case TF_C_F16_F32:
-            ((uint32_t *)dst)[4 * (uint64_t)idx + 3] = (uint32_t)__half_as_ushort(__float2half_rn(v));
-            break;
-        case TF_C_U8_F32: ((uint32_t *)dst)[4 * (uint64_t)idx + 3] = enc_u8(v); break;
-        case TF_C_F32_SN8: ((float *)dst)[2 * (uint64_t)idx] = v; break;
-        case TF_C_F16_SN8:
-            ((uint32_t *)dst)[2 * (uint64_t)idx] = (uint32_t)__half_as_ushort(__float2half_rn(v));
-            break;
-        case TF_C_U8_SN8: ((uint8_t *)dst)[4 * (uint64_t)idx + 3] = (uint8_t)enc_u8(v); break;
-        default: break;
-        }
-    } else {
-        const float nx = src[3 * l], ny = src[3 * l + 1], nz = src[3 * l + 2];
-        switch (fmt) {
-        case TF_N_F32:
-        case TF_C_F32_F32:
-        case TF_C_F16_F32:
-        case TF_C_U8_F32: {
-            float *p = (float *)dst + 4 * (uint64_t)idx;
-            p[0] = nx; p[1] = ny; p[2] = nz;
-            if (fmt == TF_N_F32) p[3] = 0.f;
-            break;
-        }
-        case TF_N_SN8:
-            ((uint32_t *)dst)[idx] = enc_sn8(nx) | (enc_sn8(ny) << 8) | (enc_sn8(nz) << 16);
-            break;
-        case TF_C_F32_SN8:
-        case TF_C_F16_SN8:
-            ((uint32_t *)dst)[2 * (uint64_t)idx + 1] = enc_sn8(nx) | (enc_sn8(ny) << 8) | (enc_sn8(nz) << 16);
-            break;
-        case TF_C_U8_SN8: {
-            uint8_t *p = (uint8_t *)dst + 4 * (uint64_t)idx;
-            p[0] = (uint8_t)enc_sn8(nx); p[1] = (uint8_t)enc_sn8(ny); p[2] = (uint8_t)enc_sn8(nz);
-            break;
-        }
-        default: break;
-        }
-    }
+    uint32_t bx[4], lx[4], by[4], ly[4], bz[4], lz[4];
+    const int nx = axis_slots(x, s0, lb, nbx, bx, lx);
+    const int ny = axis_slots(y, s1, lb, nby, by, ly);
+    const int nz = axis_slots(z, s2, lb, nbz, bz, lz);
+    const uint32_t E = (1u << lb) + 1u;
+    constexpr uint32_t T = (uint32_t)texel_bytes(F);
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const uint32_t brick = (bz[k] * nby + by[j]) * nbx + bx[i];
+                const uint32_t local = (lz[k] * E + ly[j]) * E + lx[i];
+                store_part<F>(dst + (uint64_t)brick * brick_bytes + (uint64_t)local * T, part, src, l);
+            }
 }
 
 }  // namespace hvr

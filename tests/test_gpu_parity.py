@@ -58,9 +58,10 @@ def test_deterministic(hip):
     assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("lb", [0, 2, 3])
+@pytest.mark.parametrize("lb", [2, 4])
 def test_brick_layouts_identical(hip, lb):
+    """Brick edge 4 (generic path), 8 (default) and 16 give the same bytes."""
     sc = scenes.c1(size=40, image=64)
-    base = render_hip(sc, brick_log2=0)
+    base = render_hip(sc, brick_log2=3)
     img = render_hip(sc, brick_log2=lb)
     assert np.array_equal(base, img)
