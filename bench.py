@@ -136,6 +136,8 @@ def main():
     ap.add_argument("--scalar-format", default="f16", choices=["f32", "f16", "u8"])
     ap.add_argument("--normal-format", default="sn8x3", choices=["f32x3", "sn8x3"])
     ap.add_argument("--brick-log2", type=int, default=-1)
+    ap.add_argument("--skip", choices=("auto", "on", "off"), default="auto",
+                    help="transparent-brick skipping mode (result is identical in every mode)")
     ap.add_argument("--latency-frames", type=int, default=48)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -162,6 +164,8 @@ def main():
     ctx = r.gl_manager.ctx
     if args.brick_log2 >= 0:
         _lib.check(L.hvr_set_brick_log2(ctx, args.brick_log2), "set_brick")
+    _lib.check(L.hvr_set_empty_skip(ctx, {"off": _lib.SKIP_OFF, "on": _lib.SKIP_ON,
+                                           "auto": _lib.SKIP_AUTO}[args.skip]), "set_empty_skip")
     t0 = time.perf_counter()
     r.load_volume(sc.volume())
     vol_bytes = sum(v.nbytes() for v in r.gl_manager._volumes)
@@ -220,10 +224,16 @@ def main():
     log(f"timed {args.steps} steps: {ms_step:.2f} ms/step, {value / 1e9:.2f} G samples/s; "
         f"march kernel avg {march_ms.value / nfr.value:.3f} ms, setup {setup_ms.value / nfr.value:.3f} ms")
 
+    clear_b, total_b, used = ctypes.c_ulonglong(), ctypes.c_ulonglong(), ctypes.c_int()
+    _lib.check(L.hvr_skip_info(ctx, ctypes.byref(clear_b), ctypes.byref(total_b), ctypes.byref(used)), "skip_info")
+    skip = {"mode": args.skip, "used": bool(used.value), "clear_bricks": clear_b.value, "bricks": total_b.value}
+    log(f"transparent bricks {clear_b.value}/{total_b.value}; stepped over: {bool(used.value)}")
+
     bpv = 8 * (SCALAR_BYTES[sf] + (NORMAL_BYTES[nf] if sc.normals is not None else 0))
     alg_bytes = s_step * args.steps * bpv + 4 * npix * nfr.value
     achieved = alg_bytes / (march_ms.value * 1e-3)
-    tkey = f"{sc.name}/{len(cams)}cams/{sf}+{nf if sc.normals is not None else 'none'}/lb{args.brick_log2}"
+    tkey = (f"{sc.name}/{len(cams)}cams/{sf}+{nf if sc.normals is not None else 'none'}/lb{args.brick_log2}"
+            f"/skip{int(used.value)}")
     traffic, tsrc = measured_traffic(tkey)
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK, "traffic": traffic,
@@ -271,7 +281,7 @@ def main():
                                              "max_steps": sc.config.max_steps},
             "et_threshold": sc.tau, "scalar_format": sf,
             "normal_format": nf if sc.normals is not None else "none",
-            "samples_per_step": s_step, "volume_bytes": vol_bytes,
+            "samples_per_step": s_step, "volume_bytes": vol_bytes, "brick_skip": skip,
         },
         "fps": 1e3 / float(lat.mean()),
         "frame_ms_p50": float(np.percentile(lat, 50)),

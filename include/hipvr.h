@@ -181,9 +181,27 @@ int hvr_device_copy(hvr_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 int hvr_timing_begin(hvr_ctx *ctx, int max_frames);
 int hvr_timing_end(hvr_ctx *ctx, double *march_ms, double *setup_ms, int *frames);
 
+/* Transparent-brick skipping (no counterpart in the reference).  Samples in
+ * bricks where every LUT entry that a density of the brick can weight has
+ * alpha 0 (and a finite colour) contribute nothing; the march can count
+ * them and step over them without loading texels.  The image and the sample
+ * counts are identical in every mode.  HVR_SKIP_AUTO (the default) steps
+ * over bricks when at least a quarter of the bound volume's bricks are
+ * transparent for the current LUT.  The first frame after a LUT or volume
+ * change computes the brick flags and waits for their count. */
+#define HVR_SKIP_OFF 0
+#define HVR_SKIP_ON 1
+#define HVR_SKIP_AUTO 2
+int hvr_set_empty_skip(hvr_ctx *ctx, int mode);
+
+/* Transparent bricks of the volume the last frame's flags were computed for
+ * (0 and 0 when none were), and whether the last frame's march stepped over
+ * them. */
+int hvr_skip_info(hvr_ctx *ctx, unsigned long long *clear_bricks, unsigned long long *bricks, int *used);
+
 /* Tuning knobs (no counterpart in the reference): log2 of the brick edge
- * used by volumes created afterwards (0 = linear layout, 2 = 4^3 bricks,
- * 3 = 8^3 bricks). */
+ * used by volumes created afterwards: 2, 3 or 4 (4^3, 8^3 or 16^3 bricks,
+ * each stored with a one-texel apron; default 3). */
 int hvr_set_brick_log2(hvr_ctx *ctx, int log2_edge);
 
 #ifdef __cplusplus

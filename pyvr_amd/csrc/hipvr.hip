@@ -46,6 +46,8 @@ struct hvr_volume {
     uint64_t bytes;
     void *d_texels;
     bool scalar_loaded, normals_loaded;
+    float2 *d_range;       // per-brick scalar range (after a scalar upload)
+    uint32_t gen;          // bumped on every upload
 };
 
 struct hvr_ctx {
@@ -71,7 +73,22 @@ struct hvr_ctx {
     // per-launch timing region (hvr_timing_begin/end)
     std::vector<hipEvent_t> *tev;
     int timing_max, timing_n;
+    // transparent-brick skipping
+    int *d_nz;             // prefix counts of non-transparent LUT entries (lut_n + 1)
+    uint32_t lut_gen;
+    uint8_t *d_flags;      // per-brick transparent flags of flags_vol
+    uint32_t *d_nclear;    // transparent-brick count written by k_brick_flags
+    uint64_t flags_cap;
+    const hvr_volume *flags_vol;
+    uint32_t flags_vol_gen, flags_lut_gen;
+    uint64_t flags_clear, flags_total;
+    int skip_mode;         // HVR_SKIP_OFF / _ON / _AUTO
+    int skip_used;         // whether the last frame's march stepped over bricks
 };
+
+// HVR_SKIP_AUTO steps over transparent bricks when at least this fraction of
+// the bound volume's bricks are transparent for the current LUT.
+static constexpr double kSkipAutoFraction = 0.25;
 
 
 // ---------------------------------------------------------------------------
@@ -184,23 +201,74 @@ static TexDesc tex_desc(const hvr_volume *v, int role, const FrameConst &fc)
 
 // ---------------------------------------------------------------------------
 // kernel dispatch
-template <int SF, int NM, int LB, bool COUNT>
+template <int SF, int NM, int LB, bool COUNT, bool SKIP = false>
 static void launch_march_t(hvr_ctx *c, const FrameConst &fc, const TexDesc &ts, const TexDesc &tn,
-                           int ntx, int nblocks)
+                           int ntx, int nblocks, const uint8_t *skip)
 {
     const size_t lds = NM == NM_ANY ? 0 : 2 * (size_t)c->lut_n * sizeof(float4);
-    hipLaunchKernelGGL((k_march<SF, NM, LB, COUNT>), dim3(nblocks), dim3(256), lds, c->stream, fc, ts, tn,
-                       (const float4 *)c->d_lut, c->lut_n, (const RayRec *)c->d_rays, c->d_out,
-                       c->d_counts, ntx, nblocks);
+    hipLaunchKernelGGL((k_march<SF, NM, LB, COUNT, SKIP>), dim3(nblocks), dim3(256), lds, c->stream, fc, ts,
+                       tn, (const float4 *)c->d_lut, c->lut_n, (const RayRec *)c->d_rays, c->d_out,
+                       c->d_counts, skip, ntx, nblocks);
+}
+
+// Transparent-brick flags of volume v for the current LUT, recomputed (and
+// their count read back, which waits for the stream) when either changed.
+// Returns the flags when the march should step over transparent bricks:
+// always in HVR_SKIP_ON, in HVR_SKIP_AUTO when enough bricks are
+// transparent; nullptr otherwise.
+static const uint8_t *brick_flags(hvr_ctx *c, const hvr_volume *v)
+{
+    if (c->skip_mode == HVR_SKIP_OFF || !v || !v->d_range || !c->d_nz) return nullptr;
+    const uint64_t nb = (uint64_t)v->nbx * v->nby * v->nbz;
+    if (!(c->flags_vol == v && c->flags_vol_gen == v->gen && c->flags_lut_gen == c->lut_gen)) {
+        c->flags_vol = nullptr;
+        if (!c->d_nclear && hipMalloc(&c->d_nclear, sizeof(uint32_t)) != hipSuccess) return nullptr;
+        if (nb > c->flags_cap) {
+            if (hipStreamSynchronize(c->stream) != hipSuccess) return nullptr;
+            (void)hipFree(c->d_flags);
+            c->d_flags = nullptr;
+            c->flags_cap = 0;
+            if (hipMalloc(&c->d_flags, nb) != hipSuccess) return nullptr;
+            c->flags_cap = nb;
+        }
+        if (hipMemsetAsync(c->d_nclear, 0, sizeof(uint32_t), c->stream) != hipSuccess) return nullptr;
+        hipLaunchKernelGGL(k_brick_flags, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, c->stream,
+                           (const float2 *)v->d_range, (uint32_t)nb, (const int *)c->d_nz, c->lut_n, c->d_flags,
+                           c->d_nclear);
+        uint32_t clear = 0;
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(&clear, c->d_nclear, sizeof(clear), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            return nullptr;
+        c->flags_vol = v;
+        c->flags_vol_gen = v->gen;
+        c->flags_lut_gen = c->lut_gen;
+        c->flags_clear = clear;
+        c->flags_total = nb;
+    }
+    if (c->skip_mode == HVR_SKIP_ON || (double)c->flags_clear >= kSkipAutoFraction * (double)c->flags_total)
+        return c->d_flags;
+    return nullptr;
 }
 
 template <int SF, int LB, bool COUNT>
-static void launch_fast(hvr_ctx *c, const FrameConst &fc, const TexDesc &ts, int nm, int ntx, int nblocks)
+static void launch_fast(hvr_ctx *c, const FrameConst &fc, const TexDesc &ts, const hvr_volume *S, int nm,
+                        int ntx, int nblocks)
 {
     const TexDesc none = tex_desc(nullptr, 0, fc);
-    if (nm == NM_ZERO) launch_march_t<SF, NM_ZERO, LB, COUNT>(c, fc, ts, none, ntx, nblocks);
-    else if (nm == NM_ALIAS) launch_march_t<SF, NM_ALIAS, LB, COUNT>(c, fc, ts, none, ntx, nblocks);
-    else if constexpr (fmt_has_normal(SF)) launch_march_t<SF, NM_SAME, LB, COUNT>(c, fc, ts, none, ntx, nblocks);
+    const uint8_t *skip = brick_flags(c, S);
+    c->skip_used = skip != nullptr;
+    if (skip) {
+        if (nm == NM_ZERO) launch_march_t<SF, NM_ZERO, LB, COUNT, true>(c, fc, ts, none, ntx, nblocks, skip);
+        else if (nm == NM_ALIAS) launch_march_t<SF, NM_ALIAS, LB, COUNT, true>(c, fc, ts, none, ntx, nblocks, skip);
+        else if constexpr (fmt_has_normal(SF))
+            launch_march_t<SF, NM_SAME, LB, COUNT, true>(c, fc, ts, none, ntx, nblocks, skip);
+        return;
+    }
+    if (nm == NM_ZERO) launch_march_t<SF, NM_ZERO, LB, COUNT>(c, fc, ts, none, ntx, nblocks, nullptr);
+    else if (nm == NM_ALIAS) launch_march_t<SF, NM_ALIAS, LB, COUNT>(c, fc, ts, none, ntx, nblocks, nullptr);
+    else if constexpr (fmt_has_normal(SF))
+        launch_march_t<SF, NM_SAME, LB, COUNT>(c, fc, ts, none, ntx, nblocks, nullptr);
 }
 
 template <bool COUNT>
@@ -212,8 +280,9 @@ static void launch_march(hvr_ctx *c, const FrameConst &fc)
     const TexDesc ts = tex_desc(S, c->scalar_role, fc), tn = tex_desc(N, c->normal_role, fc);
     const TexDesc none = tex_desc(nullptr, 0, fc);
     const bool small_lut = c->lut_n <= kLutLdsMax;
+    c->skip_used = 0;
     if (small_lut && !S && !N) {
-        launch_march_t<TF_NONE, NM_ZERO, 3, COUNT>(c, fc, none, none, ntx, nblocks);
+        launch_march_t<TF_NONE, NM_ZERO, 3, COUNT>(c, fc, none, none, ntx, nblocks, nullptr);
         return;
     }
     // Fast paths: scalar field of one volume, normals from the same texels,
@@ -225,8 +294,8 @@ static void launch_march(hvr_ctx *c, const FrameConst &fc)
         else if (N == S && c->normal_role == HVR_ROLE_NORMAL && fmt_has_scalar(S->fmt)) nm = NM_SAME;
 #define HVR_CASE(SF)                                                                          \
     case SF:                                                                                  \
-        if (S->lb == 3) launch_fast<SF, 3, COUNT>(c, fc, ts, nm, ntx, nblocks);               \
-        else launch_fast<SF, 4, COUNT>(c, fc, ts, nm, ntx, nblocks);                          \
+        if (S->lb == 3) launch_fast<SF, 3, COUNT>(c, fc, ts, S, nm, ntx, nblocks);            \
+        else launch_fast<SF, 4, COUNT>(c, fc, ts, S, nm, ntx, nblocks);                       \
         return;
         if (nm >= 0) {
             switch (S->fmt) {
@@ -244,7 +313,7 @@ static void launch_march(hvr_ctx *c, const FrameConst &fc)
         }
 #undef HVR_CASE
     }
-    launch_march_t<TF_NONE, NM_ANY, 3, COUNT>(c, fc, ts, tn, ntx, nblocks);
+    launch_march_t<TF_NONE, NM_ANY, 3, COUNT>(c, fc, ts, tn, ntx, nblocks, nullptr);
 }
 
 static int enqueue_frame(hvr_ctx *c, bool count, uchar4 *dst)
@@ -280,6 +349,43 @@ static int enqueue_frame(hvr_ctx *c, bool count, uchar4 *dst)
 
 // ---------------------------------------------------------------------------
 // C ABI
+template <int F, int LB>
+static void launch_range(hvr_volume *v, uint32_t nb)
+{
+    hipLaunchKernelGGL((k_brick_range<F, LB>), dim3((nb + 3) / 4), dim3(256), 0, v->ctx->stream,
+                       (const char *)v->d_texels, v->brick_bytes, nb, v->d_range);
+}
+
+// Per-brick scalar range of a volume (for transparent-brick skipping).
+static int compute_range(hvr_volume *v)
+{
+    hvr_ctx *c = v->ctx;
+    const uint64_t nb = (uint64_t)v->nbx * v->nby * v->nbz;
+    if (!v->d_range) HIPCHK(hipMalloc(&v->d_range, nb * sizeof(float2)));
+#define HVR_RG(F)                                                 \
+    case F:                                                       \
+        if (v->lb == 2) launch_range<F, 2>(v, (uint32_t)nb);      \
+        else if (v->lb == 3) launch_range<F, 3>(v, (uint32_t)nb); \
+        else launch_range<F, 4>(v, (uint32_t)nb);                 \
+        break;
+    switch (v->fmt) {
+        HVR_RG(TF_S_F32)
+        HVR_RG(TF_S_F16)
+        HVR_RG(TF_S_U8)
+        HVR_RG(TF_C_F32_F32)
+        HVR_RG(TF_C_F16_F32)
+        HVR_RG(TF_C_U8_F32)
+        HVR_RG(TF_C_F32_SN8)
+        HVR_RG(TF_C_F16_SN8)
+        HVR_RG(TF_C_U8_SN8)
+    default: return HVR_OK;
+    }
+#undef HVR_RG
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return HVR_OK;
+}
+
 extern "C" {
 
 int hvr_abi_version(void) { return HVR_ABI_VERSION; }
@@ -306,6 +412,7 @@ int hvr_create(int device, int width, int height, hvr_ctx **out)
     c->W = width;
     c->H = height;
     c->brick_log2 = 3;
+    c->skip_mode = HVR_SKIP_AUTO;
     const size_t npix = (size_t)width * height;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&c->d_out, npix * sizeof(uchar4));
@@ -339,6 +446,9 @@ int hvr_destroy(hvr_ctx *c)
     (void)hipFree(c->d_counts);
     (void)hipFree(c->d_lut);
     (void)hipFree(c->d_stage);
+    (void)hipFree(c->d_nz);
+    (void)hipFree(c->d_flags);
+    (void)hipFree(c->d_nclear);
     if (c->h_pinned) (void)hipHostFree(c->h_pinned);
     for (int i = 0; i < 4; ++i)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
@@ -348,6 +458,25 @@ int hvr_destroy(hvr_ctx *c)
     }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+    return HVR_OK;
+}
+
+int hvr_set_empty_skip(hvr_ctx *c, int mode)
+{
+    if (!c) return fail(HVR_EINVAL, "ctx is null");
+    if (mode != HVR_SKIP_OFF && mode != HVR_SKIP_ON && mode != HVR_SKIP_AUTO)
+        return fail(HVR_EINVAL, "skip mode must be HVR_SKIP_OFF, HVR_SKIP_ON or HVR_SKIP_AUTO");
+    c->skip_mode = mode;
+    return HVR_OK;
+}
+
+int hvr_skip_info(hvr_ctx *c, unsigned long long *clear_bricks, unsigned long long *bricks, int *used)
+{
+    if (!c) return fail(HVR_EINVAL, "ctx is null");
+    const bool valid = c->flags_vol != nullptr;
+    if (clear_bricks) *clear_bricks = valid ? c->flags_clear : 0;
+    if (bricks) *bricks = valid ? c->flags_total : 0;
+    if (used) *used = c->skip_used;
     return HVR_OK;
 }
 
@@ -486,8 +615,10 @@ int hvr_volume_upload_scalar(hvr_volume *v, const float *data)
     if (!v || !data) return fail(HVR_EINVAL, "null argument");
     if (!fmt_has_scalar(v->fmt)) return fail(HVR_ESTATE, "volume has no scalar field");
     int rc = upload_part(v, data, 1);
-    if (rc == HVR_OK) v->scalar_loaded = true;
-    return rc;
+    if (rc != HVR_OK) return rc;
+    v->scalar_loaded = true;
+    ++v->gen;
+    return compute_range(v);
 }
 
 int hvr_volume_upload_normals(hvr_volume *v, const float *data)
@@ -495,7 +626,10 @@ int hvr_volume_upload_normals(hvr_volume *v, const float *data)
     if (!v || !data) return fail(HVR_EINVAL, "null argument");
     if (!fmt_has_normal(v->fmt)) return fail(HVR_ESTATE, "volume has no normal field");
     int rc = upload_part(v, data, 2);
-    if (rc == HVR_OK) v->normals_loaded = true;
+    if (rc == HVR_OK) {
+        v->normals_loaded = true;
+        ++v->gen;
+    }
     return rc;
 }
 
@@ -514,7 +648,9 @@ int hvr_volume_destroy(hvr_volume *v)
     (void)hipStreamSynchronize(c->stream);
     if (c->scalar_vol == v) c->scalar_vol = nullptr;
     if (c->normal_vol == v) c->normal_vol = nullptr;
+    if (c->flags_vol == v) c->flags_vol = nullptr;
     (void)hipFree(v->d_texels);
+    (void)hipFree(v->d_range);
     delete v;
     return HVR_OK;
 }
@@ -536,7 +672,27 @@ int hvr_set_lut(hvr_ctx *c, const float *rgba, int n)
         c->lut_cap = cnt;
     }
     HIPCHK(hipMemcpy(c->d_lut, src, (size_t)cnt * sizeof(float4), hipMemcpyHostToDevice));
+    // Prefix counts of entries that are not exactly transparent.  The march
+    // reads entry i as v[i] + a*(v[i+1] - v[i]) with a in [0, 1), so entry i
+    // yields alpha 0 and a finite colour for every a (the reach of a > 0 into
+    // entry i+1 is accounted for by the brick rule) only when its alpha is 0
+    // and the colours of it and of its successor, and the successor's alpha,
+    // are bounded (|x| <= 1e38 also excludes NaN and overflow of the
+    // difference: 0 * inf is NaN).
+    auto bounded = [](float x) { return fabsf(x) <= 1e38f; };
+    std::vector<int> nz((size_t)cnt + 1, 0);
+    for (int i = 0; i < cnt; ++i) {
+        const float *e = src + 4 * i, *f = i + 1 < cnt ? e + 4 : e;
+        const bool clear = e[3] == 0.f && bounded(e[0]) && bounded(e[1]) && bounded(e[2]) && bounded(f[0]) &&
+                           bounded(f[1]) && bounded(f[2]) && bounded(f[3]);
+        nz[i + 1] = nz[i] + (clear ? 0 : 1);
+    }
+    (void)hipFree(c->d_nz);
+    c->d_nz = nullptr;
+    HIPCHK(hipMalloc(&c->d_nz, nz.size() * sizeof(int)));
+    HIPCHK(hipMemcpy(c->d_nz, nz.data(), nz.size() * sizeof(int), hipMemcpyHostToDevice));
     c->lut_n = cnt;
+    ++c->lut_gen;
     return HVR_OK;
 }
 

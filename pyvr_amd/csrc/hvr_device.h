@@ -186,6 +186,7 @@ __device__ __forceinline__ float4 fetch_any(int fmt, const char *p)
 struct Footprint {
     const char *p;    // byte address of the lower-corner texel
     float a, b, g;    // weights along texture x, y, z
+    uint32_t brick;   // brick holding the whole footprint
 };
 
 template <int T, int LB>
@@ -202,6 +203,7 @@ __device__ __forceinline__ Footprint footprint(const TexDesc &t, float w0, float
     const uint32_t brick = ((Z >> LB) * t.nby + (Y >> LB)) * t.nbx + (X >> LB);
     const uint32_t local = ((Z & M) * E + (Y & M)) * E + (X & M);
     f.p = (const char *)t.base + (uint64_t)brick * t.brick_bytes + (uint64_t)(local * (uint32_t)T);
+    f.brick = brick;
     return f;
 }
 
@@ -410,6 +412,42 @@ __device__ __forceinline__ float4 lut_sample(P val, P dlt, float fn1, float fn, 
 
 static constexpr int kLutLdsMax = 2048;  // entries; 2 x 16 B each in LDS
 
+// Clamp-to-edge LUT lookup from global memory (generic path).
+__device__ __forceinline__ float4 lut_global(const float4 *__restrict__ lut, int n, float fn1, float fn, float rho)
+{
+    const float u = fminf(fmaxf(fmaf(rho, fn, -0.5f), 0.f), fn1);
+    const float fu = floorf(u);
+    const int i0 = (int)fu, i1 = min(i0 + 1, n - 1);
+    const float a = u - fu;
+    const float4 v0 = lut[i0], v1 = lut[i1];
+    return make_float4(fmaf(a, v1.x - v0.x, v0.x), fmaf(a, v1.y - v0.y, v0.y), fmaf(a, v1.z - v0.z, v0.z),
+                       fmaf(a, v1.w - v0.w, v0.w));
+}
+
+// One sample's contribution (volume.frag.glsl:96-115): Beer-Lambert opacity
+// correction 1 - exp(-alpha*step/ref), ambient + Lambert with the zero-vector
+// rule (a normal of zero or subnormal length has no direction and gives no
+// diffuse light), front-to-back composite.
+template <int NM>
+__device__ __forceinline__ void composite(const FrameConst &fc, const float4 &q, float n0, float n1, float n2,
+                                          float &C0, float &C1, float &C2, float &A)
+{
+    const float a = 1.f - exp2f(-(q.w * fc.alpha_k));
+    float diff = 0.f;
+    if constexpr (NM != NM_ZERO) {
+        const float nn = n0 * n0 + n1 * n1 + n2 * n2;
+        const float dt = (n0 * fc.L[0] + n1 * fc.L[1] + n2 * fc.L[2]) * rsqrtf(nn);
+        diff = (fc.L_ok && nn >= 1.17549435e-38f) ? fmaxf(dt, 0.f) : 0.f;
+    }
+    const float lam = fmaf(fc.kd, diff, fc.ka);
+    const float ta = (1.f - A) * a;
+    const float tl = ta * lam;
+    C0 = fmaf(tl, q.x, C0);
+    C1 = fmaf(tl, q.y, C1);
+    C2 = fmaf(tl, q.z, C2);
+    A = A + ta;
+}
+
 // K2 -- march and composite (volume.frag.glsl:84-121), one thread per ray.
 // A 256-thread block covers 16x16 pixels; each wave64 an 8x8 tile.  Blocks
 // are spread so that each of the 8 XCDs gets a contiguous band of tiles (its
@@ -419,12 +457,27 @@ static constexpr int kLutLdsMax = 2048;  // entries; 2 x 16 B each in LDS
 //
 // In-box test (:88-92): fl((p - bmin)/ext) lies in [0, 1] exactly when
 // fl(p - bmin) lies in [0, ext] (for ext > 0 the quotient of a larger
-// difference rounds above 1), so the test needs no division.
-template <int SF, int NM, int LB, bool COUNT>
+// difference rounds above 1), so the test needs no division.  The in-box
+// samples of a ray are one contiguous run (each coordinate moves
+// monotonically), so the loop skips the leading outside samples (the entry
+// point may round outside) and ends at the first outside sample after them.
+//
+// Transparent bricks (SKIP instances): `skip[brick]` is set when every LUT
+// entry that a density of the brick can weight is transparent (alpha 0 and
+// a finite colour).  Such a sample contributes exactly nothing
+// (C += (1-A)*lambda*rgb*0, A += (1-A)*0; lambda is finite because the
+// Lambert term is 0 or max(dot, 0) with fmaxf dropping a NaN), so it is
+// counted and stepped over without loading texels.
+__device__ __forceinline__ bool inbox(const FrameConst &fc, float wx, float wy, float wz)
+{
+    return wz >= 0.f && wz <= fc.ext[2] && wy >= 0.f && wy <= fc.ext[1] && wx >= 0.f && wx <= fc.ext[0];
+}
+
+template <int SF, int NM, int LB, bool COUNT, bool SKIP>
 __global__ void __launch_bounds__(256)
 k_march(const FrameConst fc, const TexDesc ts, const TexDesc tn, const float4 *__restrict__ lut_g,
         int lut_n, const RayRec *__restrict__ rays, uchar4 *__restrict__ out,
-        uint32_t *__restrict__ counts, int ntx, int nblocks)
+        uint32_t *__restrict__ counts, const uint8_t *__restrict__ skip, int ntx, int nblocks)
 {
     extern __shared__ float4 s_lut[];
     float4 *s_val = s_lut, *s_dlt = s_lut + lut_n;
@@ -455,36 +508,38 @@ k_march(const FrameConst fc, const TexDesc ts, const TexDesc tn, const float4 *_
     float x = r.p0x, y = r.p0y, z = r.p0z;
     float C0 = 0.f, C1 = 0.f, C2 = 0.f, A = 0.f;
     uint32_t ns = 0;
-    bool entered = false;
     const float fn = (float)lut_n, fn1 = (float)(lut_n - 1);
     constexpr float u8k = fmt_u8_scalar(SF) ? 1.0f / 255.0f : 1.0f;
-    for (int i = 0; i < r.nlim; ++i) {
-        if (!(A < fc.tau)) break;
-        const float wx = x - fc.bmin[0], wy = y - fc.bmin[1], wz = z - fc.bmin[2];
-        const bool valid = wz >= 0.f && wz <= fc.ext[2] && wy >= 0.f && wy <= fc.ext[1] &&
-                           wx >= 0.f && wx <= fc.ext[0];
-        if (valid) {
-            entered = true;
+    constexpr int T = texel_bytes(SF);
+    const int nlim = r.nlim;
+    int i = 0;
+    // leading samples outside the box
+    for (; i < nlim; ++i) {
+        if (inbox(fc, x - fc.bmin[0], y - fc.bmin[1], z - fc.bmin[2])) break;
+        x = __fadd_rn(x, r.dsx);
+        y = __fadd_rn(y, r.dsy);
+        z = __fadd_rn(z, r.dsz);
+    }
+    if constexpr (!SKIP || NM == NM_ANY || SF == TF_NONE) {
+        // the in-box run
+        for (; i < nlim; ++i) {
+            if (!(A < fc.tau)) break;
+            const float wx = x - fc.bmin[0], wy = y - fc.bmin[1], wz = z - fc.bmin[2];
+            if (!inbox(fc, wx, wy, wz)) break;
             if constexpr (COUNT) ++ns;
             // texture x <- world z (size s0), y <- world y (s1), z <- world x (s2)
-            float rho, n0, n1, n2;
+            float rho = 0.f, n0 = 0.f, n1 = 0.f, n2 = 0.f;
             if constexpr (NM == NM_ANY) {
-                rho = 0.f;
                 if (ts.base) {
                     const float4 t = sample_any(ts, wz, wy, wx);
                     rho = ts.role == 0 ? t.w : t.x;
                 }
-                n0 = n1 = n2 = 0.f;
                 if (tn.base) {
                     const float4 t = sample_any(tn, wz, wy, wx);
                     if (tn.role == 1) { n0 = t.x; n1 = t.y; n2 = t.z; }
                     else n0 = t.w;
                 }
-            } else if constexpr (SF == TF_NONE) {
-                rho = 0.f;
-                n0 = n1 = n2 = 0.f;
-            } else {
-                constexpr int T = texel_bytes(SF);
+            } else if constexpr (SF != TF_NONE) {
                 const Footprint f = footprint<T, LB>(ts, wz, wy, wx);
                 if constexpr (NM == NM_SAME) {
                     const float4 t = sample4<SF, LB>(f);
@@ -493,52 +548,117 @@ k_march(const FrameConst fc, const TexDesc ts, const TexDesc tn, const float4 *_
                 } else {
                     rho = sample1<SF, LB>(f) * u8k;
                     n0 = NM == NM_ALIAS ? rho : 0.f;
-                    n1 = n2 = 0.f;
                 }
             }
-            float4 q;
-            if constexpr (NM == NM_ANY) {
-                // global LUT with clamp-to-edge taps
-                const float u = fminf(fmaxf(fmaf(rho, fn, -0.5f), 0.f), fn1);
-                const float fu = floorf(u);
-                const int i0 = (int)fu, i1 = min(i0 + 1, lut_n - 1);
-                const float aa = u - fu;
-                const float4 v0 = lut_g[i0], v1 = lut_g[i1];
-                q = make_float4(fmaf(aa, v1.x - v0.x, v0.x), fmaf(aa, v1.y - v0.y, v0.y),
-                                fmaf(aa, v1.z - v0.z, v0.z), fmaf(aa, v1.w - v0.w, v0.w));
-            } else {
-                q = lut_sample(s_val, s_dlt, fn1, fn, rho);
-            }
-            // Beer-Lambert opacity correction (:101): 1 - exp(-alpha*step/ref)
-            const float a = 1.f - exp2f(-(q.w * fc.alpha_k));
-            // Lambert term with the zero-vector rule: a normal of zero (or
-            // subnormal) length has no direction and gives no diffuse light.
-            float diff = 0.f;
-            if constexpr (NM != NM_ZERO) {
-                const float nn = n0 * n0 + n1 * n1 + n2 * n2;
-                if (fc.L_ok && nn >= 1.17549435e-38f) {
-                    const float dt = (n0 * fc.L[0] + n1 * fc.L[1] + n2 * fc.L[2]) * rsqrtf(nn);
-                    diff = fmaxf(dt, 0.f);
-                }
-            }
-            const float lam = fmaf(fc.kd, diff, fc.ka);
-            // front-to-back composite (:112-115)
-            const float ta = (1.f - A) * a;
-            const float tl = ta * lam;
-            C0 = fmaf(tl, q.x, C0);
-            C1 = fmaf(tl, q.y, C1);
-            C2 = fmaf(tl, q.z, C2);
-            A = A + ta;
-        } else if (entered) {
-            break;
+            const float4 q = NM == NM_ANY ? lut_global(lut_g, lut_n, fn1, fn, rho)
+                                          : lut_sample(s_val, s_dlt, fn1, fn, rho);
+            composite<NM>(fc, q, n0, n1, n2, C0, C1, C2, A);
+            x = __fadd_rn(x, r.dsx);
+            y = __fadd_rn(y, r.dsy);
+            z = __fadd_rn(z, r.dsz);
         }
-        // A5: incremental FP32 stepping (:118)
-        x = __fadd_rn(x, r.dsx);
-        y = __fadd_rn(y, r.dsy);
-        z = __fadd_rn(z, r.dsz);
+    } else {
+        // the in-box run with transparent bricks stepped over.  The next
+        // sample's footprint and flag are carried one iteration ahead, so
+        // the flag's load overlaps the current sample's work instead of
+        // preceding its texel loads.
+        float wx = x - fc.bmin[0], wy = y - fc.bmin[1], wz = z - fc.bmin[2];
+        bool in = i < nlim && inbox(fc, wx, wy, wz);
+        Footprint f = footprint<T, LB>(ts, in ? wz : 0.f, in ? wy : 0.f, in ? wx : 0.f);
+        bool clear = in && skip[f.brick] != 0;
+        for (; i < nlim; ++i) {
+            if (!(A < fc.tau) || !in) break;
+            if constexpr (COUNT) ++ns;
+            x = __fadd_rn(x, r.dsx);
+            y = __fadd_rn(y, r.dsy);
+            z = __fadd_rn(z, r.dsz);
+            wx = x - fc.bmin[0];
+            wy = y - fc.bmin[1];
+            wz = z - fc.bmin[2];
+            const bool in_n = inbox(fc, wx, wy, wz);
+            const Footprint f_n = footprint<T, LB>(ts, in_n ? wz : 0.f, in_n ? wy : 0.f, in_n ? wx : 0.f);
+            const bool clear_n = in_n && skip[f_n.brick] != 0;
+            if (!clear) {
+                float rho, n0, n1 = 0.f, n2 = 0.f;
+                if constexpr (NM == NM_SAME) {
+                    const float4 t = sample4<SF, LB>(f);
+                    rho = t.w * u8k;
+                    n0 = t.x; n1 = t.y; n2 = t.z;
+                } else {
+                    rho = sample1<SF, LB>(f) * u8k;
+                    n0 = NM == NM_ALIAS ? rho : 0.f;
+                }
+                const float4 q = lut_sample(s_val, s_dlt, fn1, fn, rho);
+                composite<NM>(fc, q, n0, n1, n2, C0, C1, C2, A);
+            }
+            in = in_n;
+            f = f_n;
+            clear = clear_n;
+        }
     }
     out[pix] = blend_store(fc, C0, C1, C2, A);
     if constexpr (COUNT) counts[pix] = ns;
+}
+
+// K4 -- per-brick density range of a volume's scalar field: one wave per
+// brick reduces the decoded scalar of its E^3 texels (apron included, so
+// the range covers every footprint the brick serves).  A NaN widens the
+// range to everything.  UNORM8 ranges are in decoded units (raw / 255).
+template <int F, int LB>
+__global__ void __launch_bounds__(256)
+k_brick_range(const char *__restrict__ texels, uint32_t brick_bytes, uint32_t nbricks,
+              float2 *__restrict__ range)
+{
+    const uint32_t brick = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (brick >= nbricks) return;
+    constexpr int E = (1 << LB) + 1, N = E * E * E, T = texel_bytes(F);
+    const char *base = texels + (uint64_t)brick * brick_bytes;
+    float lo = INFINITY, hi = -INFINITY;
+    bool nan = false;
+    for (int k = lane; k < N; k += 64) {
+        float v = fetch1<F>(base + (uint64_t)k * T);
+        if constexpr (fmt_u8_scalar(F)) v *= 1.0f / 255.0f;
+        nan |= v != v;
+        lo = fminf(lo, v);
+        hi = fmaxf(hi, v);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = fminf(lo, __shfl_xor(lo, off, 64));
+        hi = fmaxf(hi, __shfl_xor(hi, off, 64));
+        nan |= __shfl_xor((int)nan, off, 64) != 0;
+    }
+    if (lane == 0) range[brick] = nan ? make_float2(-INFINITY, INFINITY) : make_float2(lo, hi);
+}
+
+// K5 -- transparent-brick flags for one LUT.  A sample of density rho reads
+// u = clamp(rho*N - 1/2, 0, N-1) and weights entry floor(u) by 1 - frac(u)
+// and entry floor(u) + 1 by frac(u), so the entries with non-zero weight
+// over densities in [lo, hi] are floor(u(lo)) .. ceil(u(hi)); the range is
+// widened by a margin for the rounding of the interpolated density.  `nz`
+// is the prefix count (N + 1 values) of entries that are not exactly
+// transparent (alpha != 0, or a non-finite colour).  `n_clear` counts the
+// transparent bricks (one atomic per wave).
+__global__ void __launch_bounds__(256)
+k_brick_flags(const float2 *__restrict__ range, uint32_t nbricks, const int *__restrict__ nz, int n,
+              uint8_t *__restrict__ flags, uint32_t *__restrict__ n_clear)
+{
+    const uint32_t brick = blockIdx.x * 256 + threadIdx.x;
+    bool clear = false;
+    if (brick < nbricks) {
+        const float2 r = range[brick];
+        const float fn = (float)n, fn1 = (float)(n - 1);
+        // an interpolated density is off by a few ulps of the largest tap
+        const float m = fmaxf(fabsf(r.x), fabsf(r.y)) * fn * 1e-5f + 1e-3f;
+        const float ulo = r.x * fn - 0.5f - m;
+        const float uhi = r.y * fn - 0.5f + m;
+        const int i0 = (int)floorf(fminf(fmaxf(ulo, 0.f), fn1));
+        const int i1 = (int)ceilf(fminf(fmaxf(uhi, 0.f), fn1));
+        clear = nz[i1 + 1] - nz[i0] == 0;
+        flags[brick] = clear ? 1 : 0;
+    }
+    const uint64_t m = __ballot(clear);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(n_clear, (uint32_t)__popcll(m));
 }
 
 // K3 -- upload: convert a slab of host float32 data (numpy C order = GL

@@ -25,8 +25,12 @@ EXPORTS = (
     "hvr_render", "hvr_render_async", "hvr_read", "hvr_synchronize", "hvr_framebuffer",
     "hvr_sample_counts", "hvr_last_kernel_ms", "hvr_set_brick_log2",
     "hvr_render_async_to", "hvr_device_alloc", "hvr_device_free", "hvr_copy_to_host",
-    "hvr_timing_begin", "hvr_timing_end", "hvr_device_copy",
+    "hvr_timing_begin", "hvr_timing_end", "hvr_device_copy", "hvr_set_empty_skip",
+    "hvr_skip_info",
 )
+
+# transparent-brick skipping modes (hvr_set_empty_skip)
+SKIP_OFF, SKIP_ON, SKIP_AUTO = 0, 1, 2
 
 
 class HvrParams(ctypes.Structure):
@@ -79,6 +83,9 @@ def _declare(L):
         "hvr_sample_counts": ([vp, ctypes.POINTER(ctypes.c_uint32)], i),
         "hvr_last_kernel_ms": ([vp, fp, fp], i),
         "hvr_set_brick_log2": ([vp, i], i),
+        "hvr_set_empty_skip": ([vp, i], i),
+        "hvr_skip_info": ([vp, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_ulonglong),
+                           ctypes.POINTER(i)], i),
         "hvr_render_async_to": ([vp, vp], i),
         "hvr_device_alloc": ([vp, ctypes.c_uint64, ctypes.POINTER(vp)], i),
         "hvr_device_free": ([vp, vp], i),

@@ -65,3 +65,78 @@ def test_brick_layouts_identical(hip, lb):
     base = render_hip(sc, brick_log2=3)
     img = render_hip(sc, brick_log2=lb)
     assert np.array_equal(base, img)
+
+
+def _render_raw(r, lut, mode):
+    """One frame and its sample counts through the C ABI with LUT `lut` and
+    skip mode `mode` (camera, volume and parameters as last set on `r`)."""
+    import ctypes
+    from pyvr_amd.hip_renderer import _lib
+
+    L, ctx = _lib.lib(), r.gl_manager.ctx
+    lut = np.ascontiguousarray(lut, np.float32)
+    _lib.check(L.hvr_set_empty_skip(ctx, mode), "skip")
+    _lib.check(L.hvr_set_lut(ctx, lut.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(lut)), "lut")
+    h, w = r.height, r.width
+    img = np.zeros((h, w, 4), np.uint8)
+    cnt = np.zeros((h, w), np.uint32)
+    _lib.check(L.hvr_render(ctx, img.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))), "render")
+    _lib.check(L.hvr_sample_counts(ctx, cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))), "counts")
+    clear, total, used = ctypes.c_ulonglong(), ctypes.c_ulonglong(), ctypes.c_int()
+    _lib.check(L.hvr_skip_info(ctx, ctypes.byref(clear), ctypes.byref(total), ctypes.byref(used)), "info")
+    return img, cnt, (clear.value, total.value, used.value)
+
+
+def _prepared(sc):
+    from pyvr_amd.hip_renderer import VolumeRenderer
+
+    r = VolumeRenderer(sc.width, sc.height, config=sc.config, light=sc.light,
+                       scalar_format="f16", normal_format="sn8x3")
+    r.load_volume(sc.volume())
+    r.set_camera(sc.camera)
+    r.set_transfer_functions(sc.ctf, sc.otf, sc.lut_size)
+    r.set_opacity_threshold(sc.tau)
+    r.render()
+    return r
+
+
+@pytest.mark.parametrize("which", ["c0", "c1", "c2"])
+def test_transparent_brick_skip_is_exact(hip, which):
+    """Stepping over transparent bricks changes neither the image nor the
+    counts, and does happen on these scenes (empty space is transparent)."""
+    from pyvr_amd.hip_renderer import _lib
+
+    sc = {"c0": lambda: scenes.c0(size=64, image=96), "c1": lambda: scenes.c1(size=96, image=128),
+          "c2": lambda: scenes.c2(size=96, image=128)}[which]()
+    r = _prepared(sc)
+    try:
+        a, ca, info_on = _render_raw(r, sc.lut(), _lib.SKIP_ON)
+        b, cb, info_off = _render_raw(r, sc.lut(), _lib.SKIP_OFF)
+    finally:
+        r.close()
+    print(f"[skip] {which}: transparent bricks {info_on[0]}/{info_on[1]}", flush=True)
+    assert info_on[2] == 1 and info_on[0] > 0 and info_off[2] == 0
+    assert np.array_equal(a, b) and np.array_equal(ca, cb)
+    assert ca.sum() > 0 and a[..., 3].max() > 0
+
+
+def test_transparent_brick_skip_with_nonfinite_lut(hip):
+    """Alpha-0 LUT entries next to a NaN or infinite entry are not transparent
+    (the filtered lookup reads the neighbour even at weight 0): skipping
+    stays byte-identical to marching every sample."""
+    from pyvr_amd.hip_renderer import _lib
+
+    sc = scenes.c0(size=64, image=96)
+    base = sc.lut()
+    r = _prepared(sc)
+    try:
+        for entry, value in [(1, np.inf), (1, np.nan), (40, -np.inf)]:
+            lut = base.copy()
+            lut[:, 3] = np.where(np.arange(len(lut)) < 60, 0.0, lut[:, 3])
+            lut[entry, entry % 3] = value
+            a, ca, info_on = _render_raw(r, lut, _lib.SKIP_ON)
+            b, cb, _ = _render_raw(r, lut, _lib.SKIP_OFF)
+            print(f"[skip] entry {entry} = {value}: transparent bricks {info_on[0]}/{info_on[1]}", flush=True)
+            assert np.array_equal(a, b) and np.array_equal(ca, cb), (entry, value)
+    finally:
+        r.close()
