@@ -94,7 +94,11 @@ const char *hvr_last_error(void);
 int hvr_device_count(int *count);
 
 /* ---- context: replaces ModernGLManager.__init__ (manager.py:14-41:
- * standalone context + RGBA8 framebuffer of width x height) ---- */
+ * standalone context + RGBA8 framebuffer of width x height) ----
+ * The march reads volumes below 4 GiB through buffer loads (32-bit
+ * offsets); with the environment variable HVR_GATHER=global set when the
+ * context is created it uses 64-bit global loads, as for larger volumes
+ * (same results; for testing that path). */
 int hvr_create(int device, int width, int height, hvr_ctx **out);
 int hvr_destroy(hvr_ctx *ctx);
 

@@ -6,7 +6,9 @@
 #include "../../include/hipvr.h"
 
 #include <math.h>
+#include <cmath>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <string>
@@ -84,6 +86,7 @@ struct hvr_ctx {
     uint64_t flags_clear, flags_total;
     int skip_mode;         // HVR_SKIP_OFF / _ON / _AUTO
     int skip_used;         // whether the last frame's march stepped over bricks
+    bool force_global;     // HVR_GATHER=global at creation: 64-bit loads for every volume
 };
 
 // HVR_SKIP_AUTO steps over transparent bricks when at least this fraction of
@@ -194,6 +197,7 @@ static TexDesc tex_desc(const hvr_volume *v, int role, const FrameConst &fc)
     t.nbx = v->nbx;
     t.nby = v->nby;
     t.brick_bytes = v->brick_bytes;
+    t.nbytes32 = v->bytes <= 0xffffffffull ? (uint32_t)v->bytes : 0u;
     t.fmt = v->fmt;
     t.role = role;
     return t;
@@ -201,13 +205,13 @@ static TexDesc tex_desc(const hvr_volume *v, int role, const FrameConst &fc)
 
 // ---------------------------------------------------------------------------
 // kernel dispatch
-template <int SF, int NM, int LB, bool COUNT, bool SKIP = false>
+template <int SF, int NM, int LB, bool COUNT, bool SKIP = false, bool BUF = false>
 static void launch_march_t(hvr_ctx *c, const FrameConst &fc, const TexDesc &ts, const TexDesc &tn,
                            int ntx, int nblocks, const uint8_t *skip)
 {
     const size_t lds = NM == NM_ANY ? 0 : 2 * (size_t)c->lut_n * sizeof(float4);
-    hipLaunchKernelGGL((k_march<SF, NM, LB, COUNT, SKIP>), dim3(nblocks), dim3(256), lds, c->stream, fc, ts,
-                       tn, (const float4 *)c->d_lut, c->lut_n, (const RayRec *)c->d_rays, c->d_out,
+    hipLaunchKernelGGL((k_march<SF, NM, LB, COUNT, SKIP, BUF>), dim3(nblocks), dim3(256), lds, c->stream, fc,
+                       ts, tn, (const float4 *)c->d_lut, c->lut_n, (const RayRec *)c->d_rays, c->d_out,
                        c->d_counts, skip, ntx, nblocks);
 }
 
@@ -251,24 +255,30 @@ static const uint8_t *brick_flags(hvr_ctx *c, const hvr_volume *v)
     return nullptr;
 }
 
+template <int SF, int LB, bool COUNT, bool SKIP, bool BUF>
+static void launch_fast_nm(hvr_ctx *c, const FrameConst &fc, const TexDesc &ts, int nm, int ntx, int nblocks,
+                           const uint8_t *skip)
+{
+    const TexDesc none = tex_desc(nullptr, 0, fc);
+    if (nm == NM_ZERO) launch_march_t<SF, NM_ZERO, LB, COUNT, SKIP, BUF>(c, fc, ts, none, ntx, nblocks, skip);
+    else if (nm == NM_ALIAS) launch_march_t<SF, NM_ALIAS, LB, COUNT, SKIP, BUF>(c, fc, ts, none, ntx, nblocks, skip);
+    else if constexpr (fmt_has_normal(SF))
+        launch_march_t<SF, NM_SAME, LB, COUNT, SKIP, BUF>(c, fc, ts, none, ntx, nblocks, skip);
+}
+
+// Fast-path instance: transparent-brick skipping when the flags say so,
+// buffer loads when the volume's allocation is below 4 GiB.
 template <int SF, int LB, bool COUNT>
 static void launch_fast(hvr_ctx *c, const FrameConst &fc, const TexDesc &ts, const hvr_volume *S, int nm,
                         int ntx, int nblocks)
 {
-    const TexDesc none = tex_desc(nullptr, 0, fc);
     const uint8_t *skip = brick_flags(c, S);
     c->skip_used = skip != nullptr;
-    if (skip) {
-        if (nm == NM_ZERO) launch_march_t<SF, NM_ZERO, LB, COUNT, true>(c, fc, ts, none, ntx, nblocks, skip);
-        else if (nm == NM_ALIAS) launch_march_t<SF, NM_ALIAS, LB, COUNT, true>(c, fc, ts, none, ntx, nblocks, skip);
-        else if constexpr (fmt_has_normal(SF))
-            launch_march_t<SF, NM_SAME, LB, COUNT, true>(c, fc, ts, none, ntx, nblocks, skip);
-        return;
-    }
-    if (nm == NM_ZERO) launch_march_t<SF, NM_ZERO, LB, COUNT>(c, fc, ts, none, ntx, nblocks, nullptr);
-    else if (nm == NM_ALIAS) launch_march_t<SF, NM_ALIAS, LB, COUNT>(c, fc, ts, none, ntx, nblocks, nullptr);
-    else if constexpr (fmt_has_normal(SF))
-        launch_march_t<SF, NM_SAME, LB, COUNT>(c, fc, ts, none, ntx, nblocks, nullptr);
+    const bool buf = ts.nbytes32 != 0 && !c->force_global;
+    if (skip && buf) launch_fast_nm<SF, LB, COUNT, true, true>(c, fc, ts, nm, ntx, nblocks, skip);
+    else if (skip) launch_fast_nm<SF, LB, COUNT, true, false>(c, fc, ts, nm, ntx, nblocks, skip);
+    else if (buf) launch_fast_nm<SF, LB, COUNT, false, true>(c, fc, ts, nm, ntx, nblocks, nullptr);
+    else launch_fast_nm<SF, LB, COUNT, false, false>(c, fc, ts, nm, ntx, nblocks, nullptr);
 }
 
 template <bool COUNT>
@@ -413,6 +423,8 @@ int hvr_create(int device, int width, int height, hvr_ctx **out)
     c->H = height;
     c->brick_log2 = 3;
     c->skip_mode = HVR_SKIP_AUTO;
+    const char *gather = getenv("HVR_GATHER");
+    c->force_global = gather && strcmp(gather, "global") == 0;
     const size_t npix = (size_t)width * height;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&c->d_out, npix * sizeof(uchar4));
@@ -701,6 +713,13 @@ int hvr_set_params(hvr_ctx *c, const hvr_params *p)
     if (!c || !p) return fail(HVR_EINVAL, "null argument");
     if (!(p->step > 0.f) || !(p->ref_step > 0.f)) return fail(HVR_EINVAL, "step sizes must be positive");
     if (p->max_steps < 0) return fail(HVR_EINVAL, "max_steps must be >= 0");
+    for (int k = 0; k < 3; ++k) {
+        // the float extent of the in-box test must be positive, with the
+        // texel scale (size / extent) finite
+        const float ext = p->bmax[k] - p->bmin[k];
+        if (!std::isfinite(p->bmin[k]) || !std::isfinite(p->bmax[k]) || !(ext >= 1e-20f) || !std::isfinite(ext))
+            return fail(HVR_EINVAL, "box bounds must be finite with bmax - bmin >= 1e-20 on every axis");
+    }
     c->params = *p;
     c->params_set = true;
     return HVR_OK;

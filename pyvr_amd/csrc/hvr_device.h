@@ -81,6 +81,7 @@ struct TexDesc {
     int lb;                // log2 of the brick interior edge
     uint32_t nbx, nby;     // bricks along x and y
     uint32_t brick_bytes;  // E^3 * texel bytes
+    uint32_t nbytes32;     // allocation bytes when below 4 GiB (buffer-load path), else 0
     int fmt;
     int role;              // field a sampler reads: 0 scalar, 1 normal
 };
@@ -184,27 +185,163 @@ __device__ __forceinline__ float4 fetch_any(int fmt, const char *p)
 // world units from the box minimum along the axes sampled by texture x, y, z
 // (world z, y, x: the shader's swizzle, volume.frag.glsl:90).
 struct Footprint {
-    const char *p;    // byte address of the lower-corner texel
-    float a, b, g;    // weights along texture x, y, z
+    const char *p;    // byte address of the lower-corner texel (global-load path)
+    uint32_t off;     // its byte offset in the allocation (buffer-load path)
+    float u, v, w;    // extended texel coordinates along texture x, y, z (>= 0 in
+                      // the box; the weights are their fractional parts)
     uint32_t brick;   // brick holding the whole footprint
 };
 
-template <int T, int LB>
+template <int T, int LB, bool BUF>
 __device__ __forceinline__ Footprint footprint(const TexDesc &t, float w0, float w1, float w2)
 {
     const float u = fmaf(w0, t.k0, 0.5f), v = fmaf(w1, t.k1, 0.5f), w = fmaf(w2, t.k2, 0.5f);
     const float fu = floorf(u), fv = floorf(v), fw = floorf(w);
     Footprint f;
-    f.a = u - fu;
-    f.b = v - fv;
-    f.g = w - fw;
+    f.u = u;
+    f.v = v;
+    f.w = w;
     const uint32_t X = (uint32_t)(int)fu, Y = (uint32_t)(int)fv, Z = (uint32_t)(int)fw;
     constexpr uint32_t M = (1u << LB) - 1u, E = (1u << LB) + 1u;
     const uint32_t brick = ((Z >> LB) * t.nby + (Y >> LB)) * t.nbx + (X >> LB);
     const uint32_t local = ((Z & M) * E + (Y & M)) * E + (X & M);
-    f.p = (const char *)t.base + (uint64_t)brick * t.brick_bytes + (uint64_t)(local * (uint32_t)T);
+    if constexpr (BUF) {
+        f.off = brick * t.brick_bytes + local * (uint32_t)T;   // < nbytes32 < 2^32
+        f.p = nullptr;
+    } else {
+        f.p = (const char *)t.base + (uint64_t)brick * t.brick_bytes + (uint64_t)(local * (uint32_t)T);
+        f.off = 0;
+    }
     f.brick = brick;
     return f;
+}
+
+// Where the march reads texels from: a buffer resource (base and size in
+// scalar registers, 32-bit per-lane offsets, reads past the allocation
+// return 0) for allocations below 4 GiB, else a 64-bit base pointer.
+template <bool BUF>
+struct Src;
+template <>
+struct Src<true> {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ explicit Src(const TexDesc &t)
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(t.base), (short)0, (int)t.nbytes32,
+                                              0x00020000)) {}
+};
+template <>
+struct Src<false> {
+    __device__ explicit Src(const TexDesc &) {}
+};
+
+typedef uint32_t hvr_u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t hvr_u32x4 __attribute__((ext_vector_type(4)));
+
+// NB dwords at byte offset o from the footprint's lower corner (o a
+// compile-time constant; the buffer path folds it into the instruction).
+template <int NB, bool BUF>
+__device__ __forceinline__ void load_words(const Src<BUF> &s, const Footprint &f, uint32_t o, uint32_t *w)
+{
+    if constexpr (BUF) {
+        if constexpr (NB == 1) {
+            w[0] = __builtin_amdgcn_raw_buffer_load_b32(s.r, f.off + o, 0, 0);
+        } else if constexpr (NB == 2) {
+            const hvr_u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(s.r, f.off + o, 0, 0);
+            w[0] = v.x; w[1] = v.y;
+        } else {
+#pragma unroll
+            for (int q = 0; q < NB / 4; ++q) {
+                const hvr_u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(s.r, f.off + o + 16u * q, 0, 0);
+                w[4 * q] = v.x; w[4 * q + 1] = v.y; w[4 * q + 2] = v.z; w[4 * q + 3] = v.w;
+            }
+        }
+    } else {
+        // texels are 4-byte aligned: load with that alignment
+        typedef hvr_u32x2 u2a __attribute__((aligned(4)));
+        typedef hvr_u32x4 u4a __attribute__((aligned(4)));
+        const char *p = f.p + o;
+        if constexpr (NB == 1) {
+            w[0] = *(const uint32_t *)p;
+        } else if constexpr (NB == 2) {
+            const hvr_u32x2 v = *(const u2a *)p;
+            w[0] = v.x; w[1] = v.y;
+        } else {
+#pragma unroll
+            for (int q = 0; q < NB / 4; ++q) {
+                const hvr_u32x4 v = *(const u4a *)(p + 16 * q);
+                w[4 * q] = v.x; w[4 * q + 1] = v.y; w[4 * q + 2] = v.z; w[4 * q + 3] = v.w;
+            }
+        }
+    }
+}
+
+// Decode one texel held in dwords w[0 .. T/4-1] (formats of 4 bytes and
+// more) to (nx, ny, nz, rho), raw SNORM8/UNORM8 integers as fetch4 gives.
+template <int F>
+__device__ __forceinline__ float4 decode_words(const uint32_t *w)
+{
+    if constexpr (F == TF_S_F32) {
+        const float v = __uint_as_float(w[0]);
+        return make_float4(v, 0.f, 0.f, v);
+    } else if constexpr (F == TF_N_F32) {
+        const float x = __uint_as_float(w[0]);
+        return make_float4(x, __uint_as_float(w[1]), __uint_as_float(w[2]), x);
+    } else if constexpr (F == TF_N_SN8) {
+        const float x = sn8f(w[0], 0);
+        return make_float4(x, sn8f(w[0], 8), sn8f(w[0], 16), x);
+    } else if constexpr (F == TF_C_F32_F32) {
+        return make_float4(__uint_as_float(w[0]), __uint_as_float(w[1]), __uint_as_float(w[2]),
+                           __uint_as_float(w[3]));
+    } else if constexpr (F == TF_C_F16_F32) {
+        return make_float4(__uint_as_float(w[0]), __uint_as_float(w[1]), __uint_as_float(w[2]),
+                           __half2float(__ushort_as_half((unsigned short)(w[3] & 0xffffu))));
+    } else if constexpr (F == TF_C_U8_F32) {
+        return make_float4(__uint_as_float(w[0]), __uint_as_float(w[1]), __uint_as_float(w[2]),
+                           (float)(w[3] & 0xffu));
+    } else if constexpr (F == TF_C_F32_SN8) {
+        return make_float4(sn8f(w[1], 0), sn8f(w[1], 8), sn8f(w[1], 16), __uint_as_float(w[0]));
+    } else if constexpr (F == TF_C_F16_SN8) {
+        return make_float4(sn8f(w[1], 0), sn8f(w[1], 8), sn8f(w[1], 16),
+                           __half2float(__ushort_as_half((unsigned short)(w[0] & 0xffffu))));
+    } else if constexpr (F == TF_C_U8_SN8) {
+        return make_float4(sn8f(w[0], 0), sn8f(w[0], 8), sn8f(w[0], 16), (float)(w[0] >> 24));
+    } else {
+        return make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+// The eight taps of a footprint, x fastest: tap i at (i&1, (i>>1)&1, i>>2).
+// Formats of 4 bytes and more load each x-adjacent pair with one load of
+// 2T bytes (texels are dword aligned); 1- and 2-byte formats load per tap.
+template <int F, int LB, bool BUF>
+__device__ __forceinline__ void load_taps(const Src<BUF> &s, const Footprint &f, float4 t[8])
+{
+    constexpr int T = texel_bytes(F);
+    constexpr uint32_t E = (1u << LB) + 1u;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t o = ((uint32_t)(k & 1) * E + (uint32_t)(k >> 1) * E * E) * (uint32_t)T;
+        if constexpr (T >= 4) {
+            uint32_t w[T / 2];
+            load_words<T / 2, BUF>(s, f, o, w);
+            t[2 * k] = decode_words<F>(w);
+            t[2 * k + 1] = decode_words<F>(w + T / 4);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                float v;
+                if constexpr (BUF) {
+                    if constexpr (T == 2)
+                        v = __half2float(__ushort_as_half(
+                            __builtin_amdgcn_raw_buffer_load_b16(s.r, f.off + o + 2u * j, 0, 0)));
+                    else
+                        v = (float)__builtin_amdgcn_raw_buffer_load_b8(s.r, f.off + o + (uint32_t)j, 0, 0);
+                } else {
+                    v = fetch1<F>(f.p + o + T * j);
+                }
+                t[2 * k + j] = make_float4(v, 0.f, 0.f, v);
+            }
+        }
+    }
 }
 
 // The eight trilinear weights, x fastest: w[i] for tap (i&1, (i>>1)&1, i>>2).
@@ -218,40 +355,42 @@ __device__ __forceinline__ void tri_weights(float a, float b, float g, float w[8
     w[6] = a1 * w11; w[7] = a * w11;
 }
 
-template <int F, int LB>
-__device__ __forceinline__ float4 sample4(const Footprint &f)
+// Weights of a footprint: frac(u) = u - floor(u), exact for u >= 0 (the
+// box); v_fract_f32 computes it in one instruction.
+__device__ __forceinline__ void fp_weights(const Footprint &f, float w[8])
 {
-    constexpr int T = texel_bytes(F), E = (1 << LB) + 1;
+    tri_weights(__builtin_amdgcn_fractf(f.u), __builtin_amdgcn_fractf(f.v), __builtin_amdgcn_fractf(f.w), w);
+}
+
+template <int F, int LB, bool BUF>
+__device__ __forceinline__ float4 sample4(const Src<BUF> &s, const Footprint &f)
+{
     float w[8];
-    tri_weights(f.a, f.b, f.g, w);
-    const float4 t0 = fetch4<F>(f.p), t1 = fetch4<F>(f.p + T);
-    const float4 t2 = fetch4<F>(f.p + E * T), t3 = fetch4<F>(f.p + (E + 1) * T);
-    const float4 t4 = fetch4<F>(f.p + E * E * T), t5 = fetch4<F>(f.p + (E * E + 1) * T);
-    const float4 t6 = fetch4<F>(f.p + (E * E + E) * T), t7 = fetch4<F>(f.p + (E * E + E + 1) * T);
+    float4 t[8];
+    load_taps<F, LB, BUF>(s, f, t);
+    fp_weights(f, w);
     float4 r;
-    r.x = w[0] * t0.x; r.y = w[0] * t0.y; r.z = w[0] * t0.z; r.w = w[0] * t0.w;
-#define HVR_ACC(k, tk) r.x = fmaf(w[k], tk.x, r.x); r.y = fmaf(w[k], tk.y, r.y); \
-                       r.z = fmaf(w[k], tk.z, r.z); r.w = fmaf(w[k], tk.w, r.w)
-    HVR_ACC(1, t1); HVR_ACC(2, t2); HVR_ACC(3, t3); HVR_ACC(4, t4);
-    HVR_ACC(5, t5); HVR_ACC(6, t6); HVR_ACC(7, t7);
-#undef HVR_ACC
+    r.x = w[0] * t[0].x; r.y = w[0] * t[0].y; r.z = w[0] * t[0].z; r.w = w[0] * t[0].w;
+#pragma unroll
+    for (int k = 1; k < 8; ++k) {
+        r.x = fmaf(w[k], t[k].x, r.x);
+        r.y = fmaf(w[k], t[k].y, r.y);
+        r.z = fmaf(w[k], t[k].z, r.z);
+        r.w = fmaf(w[k], t[k].w, r.w);
+    }
     return r;
 }
 
-template <int F, int LB>
-__device__ __forceinline__ float sample1(const Footprint &f)
+template <int F, int LB, bool BUF>
+__device__ __forceinline__ float sample1(const Src<BUF> &s, const Footprint &f)
 {
-    constexpr int T = texel_bytes(F), E = (1 << LB) + 1;
     float w[8];
-    tri_weights(f.a, f.b, f.g, w);
-    float r = w[0] * fetch1<F>(f.p);
-    r = fmaf(w[1], fetch1<F>(f.p + T), r);
-    r = fmaf(w[2], fetch1<F>(f.p + E * T), r);
-    r = fmaf(w[3], fetch1<F>(f.p + (E + 1) * T), r);
-    r = fmaf(w[4], fetch1<F>(f.p + E * E * T), r);
-    r = fmaf(w[5], fetch1<F>(f.p + (E * E + 1) * T), r);
-    r = fmaf(w[6], fetch1<F>(f.p + (E * E + E) * T), r);
-    r = fmaf(w[7], fetch1<F>(f.p + (E * E + E + 1) * T), r);
+    float4 t[8];
+    load_taps<F, LB, BUF>(s, f, t);
+    fp_weights(f, w);
+    float r = w[0] * t[0].w;
+#pragma unroll
+    for (int k = 1; k < 8; ++k) r = fmaf(w[k], t[k].w, r);
     return r;
 }
 
@@ -432,7 +571,9 @@ template <int NM>
 __device__ __forceinline__ void composite(const FrameConst &fc, const float4 &q, float n0, float n1, float n2,
                                           float &C0, float &C1, float &C2, float &A)
 {
-    const float a = 1.f - exp2f(-(q.w * fc.alpha_k));
+    // v_exp_f32 without the subnormal-result rescaling of exp2f: when
+    // exp2(-x) is subnormal, 1 - exp2(-x) rounds to 1 either way
+    const float a = 1.f - __builtin_amdgcn_exp2f(-(q.w * fc.alpha_k));
     float diff = 0.f;
     if constexpr (NM != NM_ZERO) {
         const float nn = n0 * n0 + n1 * n1 + n2 * n2;
@@ -470,10 +611,10 @@ __device__ __forceinline__ void composite(const FrameConst &fc, const float4 &q,
 // counted and stepped over without loading texels.
 __device__ __forceinline__ bool inbox(const FrameConst &fc, float wx, float wy, float wz)
 {
-    return wz >= 0.f && wz <= fc.ext[2] && wy >= 0.f && wy <= fc.ext[1] && wx >= 0.f && wx <= fc.ext[0];
+    return (wz >= 0.f) & (wz <= fc.ext[2]) & (wy >= 0.f) & (wy <= fc.ext[1]) & (wx >= 0.f) & (wx <= fc.ext[0]);
 }
 
-template <int SF, int NM, int LB, bool COUNT, bool SKIP>
+template <int SF, int NM, int LB, bool COUNT, bool SKIP, bool BUF>
 __global__ void __launch_bounds__(256)
 k_march(const FrameConst fc, const TexDesc ts, const TexDesc tn, const float4 *__restrict__ lut_g,
         int lut_n, const RayRec *__restrict__ rays, uchar4 *__restrict__ out,
@@ -520,8 +661,8 @@ k_march(const FrameConst fc, const TexDesc ts, const TexDesc tn, const float4 *_
         y = __fadd_rn(y, r.dsy);
         z = __fadd_rn(z, r.dsz);
     }
-    if constexpr (!SKIP || NM == NM_ANY || SF == TF_NONE) {
-        // the in-box run
+    if constexpr (NM == NM_ANY || SF == TF_NONE) {
+        // the in-box run, generic texture access
         for (; i < nlim; ++i) {
             if (!(A < fc.tau)) break;
             const float wx = x - fc.bmin[0], wy = y - fc.bmin[1], wz = z - fc.bmin[2];
@@ -539,16 +680,6 @@ k_march(const FrameConst fc, const TexDesc ts, const TexDesc tn, const float4 *_
                     if (tn.role == 1) { n0 = t.x; n1 = t.y; n2 = t.z; }
                     else n0 = t.w;
                 }
-            } else if constexpr (SF != TF_NONE) {
-                const Footprint f = footprint<T, LB>(ts, wz, wy, wx);
-                if constexpr (NM == NM_SAME) {
-                    const float4 t = sample4<SF, LB>(f);
-                    rho = t.w * u8k;
-                    n0 = t.x; n1 = t.y; n2 = t.z;
-                } else {
-                    rho = sample1<SF, LB>(f) * u8k;
-                    n0 = NM == NM_ALIAS ? rho : 0.f;
-                }
             }
             const float4 q = NM == NM_ANY ? lut_global(lut_g, lut_n, fn1, fn, rho)
                                           : lut_sample(s_val, s_dlt, fn1, fn, rho);
@@ -558,42 +689,54 @@ k_march(const FrameConst fc, const TexDesc ts, const TexDesc tn, const float4 *_
             z = __fadd_rn(z, r.dsz);
         }
     } else {
-        // the in-box run with transparent bricks stepped over.  The next
-        // sample's footprint and flag are carried one iteration ahead, so
-        // the flag's load overlaps the current sample's work instead of
-        // preceding its texel loads.
-        float wx = x - fc.bmin[0], wy = y - fc.bmin[1], wz = z - fc.bmin[2];
-        bool in = i < nlim && inbox(fc, wx, wy, wz);
-        Footprint f = footprint<T, LB>(ts, in ? wz : 0.f, in ? wy : 0.f, in ? wx : 0.f);
-        bool clear = in && skip[f.brick] != 0;
-        for (; i < nlim; ++i) {
-            if (!(A < fc.tau) || !in) break;
-            if constexpr (COUNT) ++ns;
-            x = __fadd_rn(x, r.dsx);
-            y = __fadd_rn(y, r.dsy);
-            z = __fadd_rn(z, r.dsz);
-            wx = x - fc.bmin[0];
-            wy = y - fc.bmin[1];
-            wz = z - fc.bmin[2];
-            const bool in_n = inbox(fc, wx, wy, wz);
-            const Footprint f_n = footprint<T, LB>(ts, in_n ? wz : 0.f, in_n ? wy : 0.f, in_n ? wx : 0.f);
-            const bool clear_n = in_n && skip[f_n.brick] != 0;
-            if (!clear) {
-                float rho, n0, n1 = 0.f, n2 = 0.f;
-                if constexpr (NM == NM_SAME) {
-                    const float4 t = sample4<SF, LB>(f);
-                    rho = t.w * u8k;
-                    n0 = t.x; n1 = t.y; n2 = t.z;
-                } else {
-                    rho = sample1<SF, LB>(f) * u8k;
-                    n0 = NM == NM_ALIAS ? rho : 0.f;
-                }
-                const float4 q = lut_sample(s_val, s_dlt, fn1, fn, rho);
-                composite<NM>(fc, q, n0, n1, n2, C0, C1, C2, A);
+        const Src<BUF> src(ts);
+        // one sample's contribution from its footprint
+        auto shade = [&](const Footprint &f) {
+            float rho, n0, n1 = 0.f, n2 = 0.f;
+            if constexpr (NM == NM_SAME) {
+                const float4 t = sample4<SF, LB, BUF>(src, f);
+                rho = t.w * u8k;
+                n0 = t.x; n1 = t.y; n2 = t.z;
+            } else {
+                rho = sample1<SF, LB, BUF>(src, f) * u8k;
+                n0 = NM == NM_ALIAS ? rho : 0.f;
             }
-            in = in_n;
-            f = f_n;
-            clear = clear_n;
+            const float4 q = lut_sample(s_val, s_dlt, fn1, fn, rho);
+            composite<NM>(fc, q, n0, n1, n2, C0, C1, C2, A);
+        };
+        if constexpr (!SKIP) {
+            // the in-box run
+            for (; i < nlim; ++i) {
+                if (!(A < fc.tau)) break;
+                const float wx = x - fc.bmin[0], wy = y - fc.bmin[1], wz = z - fc.bmin[2];
+                if (!inbox(fc, wx, wy, wz)) break;
+                if constexpr (COUNT) ++ns;
+                shade(footprint<T, LB, BUF>(ts, wz, wy, wx));
+                x = __fadd_rn(x, r.dsx);
+                y = __fadd_rn(y, r.dsy);
+                z = __fadd_rn(z, r.dsz);
+            }
+        } else {
+            // The in-box run with transparent bricks stepped over: a sample
+            // in a transparent brick is counted without loading texels.
+            float wx = x - fc.bmin[0], wy = y - fc.bmin[1], wz = z - fc.bmin[2];
+            bool in = i < nlim && inbox(fc, wx, wy, wz);
+            Footprint f = footprint<T, LB, BUF>(ts, in ? wz : 0.f, in ? wy : 0.f, in ? wx : 0.f);
+            bool clear = in && skip[f.brick] != 0;
+            while (in && A < fc.tau) {
+                if (!clear) shade(f);
+                ++i;
+                if constexpr (COUNT) ++ns;
+                x = __fadd_rn(x, r.dsx);
+                y = __fadd_rn(y, r.dsy);
+                z = __fadd_rn(z, r.dsz);
+                wx = x - fc.bmin[0];
+                wy = y - fc.bmin[1];
+                wz = z - fc.bmin[2];
+                in = i < nlim && inbox(fc, wx, wy, wz);
+                f = footprint<T, LB, BUF>(ts, in ? wz : 0.f, in ? wy : 0.f, in ? wx : 0.f);
+                clear = in && skip[f.brick] != 0;
+            }
         }
     }
     out[pix] = blend_store(fc, C0, C1, C2, A);

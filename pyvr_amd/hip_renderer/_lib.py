@@ -9,7 +9,8 @@ import ctypes
 import os
 
 _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(_PKG, "libhipvr.so")
+# HVR_LIBRARY names another build of the same ABI (for comparing builds)
+LIB_PATH = os.environ.get("HVR_LIBRARY") or os.path.join(_PKG, "libhipvr.so")
 
 HVR_OK = 0
 HVR_SCALAR = {"none": 0, "f32": 1, "f16": 2, "u8": 3}

@@ -58,6 +58,26 @@ def test_deterministic(hip):
     assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("mode", ["on", "off"])
+def test_global_and_buffer_gather_identical(hip, monkeypatch, mode):
+    """The 64-bit global-load instance (used for volumes of 4 GiB and more)
+    and the buffer-load instance give the same bytes and counts."""
+    from pyvr_amd.hip_renderer import _lib
+
+    sc = scenes.c2(size=80, image=96)
+    m = {"on": _lib.SKIP_ON, "off": _lib.SKIP_OFF}[mode]
+    out = []
+    for gather in ("", "global"):
+        monkeypatch.setenv("HVR_GATHER", gather)
+        r = _prepared(sc)
+        try:
+            out.append(_render_raw(r, sc.lut(), m)[:2])
+        finally:
+            r.close()
+    (a, ca), (b, cb) = out
+    assert np.array_equal(a, b) and np.array_equal(ca, cb)
+
+
 @pytest.mark.parametrize("lb", [2, 4])
 def test_brick_layouts_identical(hip, lb):
     """Brick edge 4 (generic path), 8 (default) and 16 give the same bytes."""
