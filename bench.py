@@ -124,12 +124,155 @@ def cpu_baseline(sc, cams, args, sf, nf):
     }
 
 
+def bench_batch(args):
+    """c3: one batch of ``--requests`` requests (FP16 volumes of 64^3,
+    128^3, 256^3 and 96x160x224 with SNORM8x3 normals, images of 256^2 to
+    1024^2, own cameras and transfer functions) rendered by one persistent
+    graph-captured launch per run, against the same requests rendered one
+    per launch.  A step is one run of the whole batch, images copied to
+    host memory.  Plain figures, same requests and resident volumes:
+    plain_kernel -- back-to-back single-request launches on one stream
+    (LUT upload, parameters, ray setup and march per request; one wait at
+    the end); plain_api -- what a renderer user gets per frame (the same
+    plus the synchronous copy of each image to the host)."""
+    import scenes
+    from pyvr_amd.hip_renderer import BatchRenderer, _lib
+    from pyvr_amd.hip_renderer.batch import request_params, rgba_lut
+    from pyvr_amd.hip_renderer.manager import _DeviceVolume
+
+    sf, nf = args.scalar_format, args.normal_format
+    lb = args.brick_log2 if args.brick_log2 >= 0 else 3
+    t0 = time.perf_counter()
+    vols = scenes.c3_volumes()
+    reqs = scenes.c3_requests(args.requests)
+    log(f"c3: {len(vols)} volumes, {len(reqs)} requests generated in {time.perf_counter() - t0:.1f}s")
+    L = _lib.lib()
+
+    b = BatchRenderer(scalar_format=sf, normal_format=nf, brick_log2=lb, tile_shape=args.tile_shape, pipeline=2)
+    for q in reqs:
+        b.add(vols[q.volume], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
+    b.run(count=True)
+    samples = sum(int(b.counts(i).sum()) for i in range(len(reqs)))
+    pixels = sum(q.width * q.height for q in reqs)
+    log(f"batch: {samples} samples, {pixels} pixels per run; resident {b.resident_bytes() / 2**20:.0f} MiB")
+    # one batch at a time: launch, wait for the images in host memory
+    for _ in range(args.warmup):
+        b.run()
+    dev_ms = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        b.run()
+        dev_ms.append(b.last_ms())
+    wall_sync = time.perf_counter() - t_start
+    value_sync = samples * args.steps / wall_sync
+    log(f"batched, one at a time: {wall_sync * 1e3 / args.steps:.2f} ms/run, {value_sync / 1e9:.2f} G samples/s "
+        f"(device {np.mean(dev_ms):.2f} ms/run)")
+    # two batches in flight (own streams): a run's copy back and tail
+    # overlap the next run's work; every run still ends with its images in
+    # host memory
+    s0 = b.submit()
+    for q in reqs:
+        b.add(vols[q.volume], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
+    s1 = b.submit()
+    b.wait(s0)
+    b.wait(s1)
+    for k in range(args.warmup):
+        b.rerun((s0, s1)[k % 2])
+    b.wait(s0)
+    b.wait(s1)
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        b.rerun((s0, s1)[k % 2])
+    b.wait(s0)
+    b.wait(s1)
+    wall = time.perf_counter() - t_start
+    value = samples * args.steps / wall
+    log(f"batched, two in flight: {wall * 1e3 / args.steps:.2f} ms/run, {value / 1e9:.2f} G samples/s")
+
+    # plain: one context per image size, every volume resident in each
+    ctxs, dvols = {}, {}
+    for q in reqs:
+        key = (q.width, q.height)
+        if key in ctxs:
+            continue
+        c = ctypes.c_void_p()
+        _lib.check(L.hvr_create(0, q.width, q.height, ctypes.byref(c)), "hvr_create")
+        _lib.check(L.hvr_set_brick_log2(c, lb), "set_brick")
+        ctxs[key] = c
+        for k, v in enumerate(vols):
+            dv = _DeviceVolume(L, c, v.data.shape, sf, nf if v.has_normals else "none")
+            dv.upload_scalar(v.data)
+            if v.has_normals:
+                dv.upload_normals(v.normals)
+            dvols[key, k] = dv
+    bufs = {}
+    for key, c in ctxs.items():
+        p = ctypes.c_void_p()
+        _lib.check(L.hvr_device_alloc(c, key[0] * key[1] * 4, ctypes.byref(p)), "alloc")
+        bufs[key] = p
+    plan = []
+    for q in reqs:
+        key = (q.width, q.height)
+        lut = rgba_lut(q.ctf, q.otf)
+        prm = request_params(vols[q.volume], q.camera, q.config, q.light, q.width, q.height, q.tau)
+        plan.append((ctxs[key], dvols[key, q.volume].handle, lut, prm, bufs[key],
+                     np.empty(q.width * q.height * 4, np.uint8)))
+
+    def plain(read):
+        for c, vh, lut, prm, buf, host in plan:
+            nrole = _lib.HVR_ROLE_NORMAL
+            _lib.check(L.hvr_bind(c, vh, _lib.HVR_ROLE_SCALAR, vh, nrole), "bind")
+            _lib.check(L.hvr_set_lut(c, lut.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), lut.shape[0]), "lut")
+            _lib.check(L.hvr_set_params(c, ctypes.byref(prm)), "params")
+            if read:
+                _lib.check(L.hvr_render(c, host.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))), "render")
+            else:
+                _lib.check(L.hvr_render_async_to(c, buf), "render")
+        for c in ctxs.values():
+            _lib.check(L.hvr_synchronize(c), "sync")
+
+    res = {}
+    for name, read in (("plain_kernel", False), ("plain_api", True)):
+        for _ in range(max(1, args.warmup)):
+            plain(read)
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            plain(read)
+        dt = time.perf_counter() - t
+        res[name] = samples * args.steps / dt
+        log(f"{name}: {dt * 1e3 / args.steps:.2f} ms/run, {res[name] / 1e9:.2f} G samples/s")
+    for key, c in ctxs.items():
+        L.hvr_device_free(c, bufs[key])
+    for dv in dvols.values():
+        dv.destroy()
+    for c in ctxs.values():
+        L.hvr_destroy(c)
+    b.close()
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"c3: {len(reqs)} batched requests (FP16 blob volumes 64^3-256^3 and "
+                               f"96x160x224, images 256^2-1024^2), one persistent graph launch per run",
+                   "requests": len(reqs), "pixels_per_step": pixels, "samples_per_step": samples,
+                   "scalar_format": sf, "normal_format": nf, "brick_log2": lb, "tile_shape": args.tile_shape},
+        "batch_device_ms": float(np.mean(dev_ms)), "batched_one_at_a_time": value_sync,
+        "plain_kernel": res["plain_kernel"], "plain_api": res["plain_api"],
+        "batched_vs_plain_kernel": value / res["plain_kernel"],
+        "batched_vs_plain_api": value / res["plain_api"],
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2", choices=["c0", "c1", "c2"])
+    ap.add_argument("--workload", default="c2", choices=["c0", "c1", "c2", "c3"])
+    ap.add_argument("--requests", type=int, default=64, help="c3: requests per batch")
+    ap.add_argument("--tile-shape", default="8x8", choices=["8x8", "16x4"], help="c3: ray tile of a wavefront")
     ap.add_argument("--size", type=int, default=0)
     ap.add_argument("--image", type=int, default=0)
     ap.add_argument("--cameras", type=int, default=24)
@@ -150,6 +293,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); this renderer's "
               "multi-GPU split is a design note, not a measured path.", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "c3":
+        return bench_batch(args)
 
     from pyvr_amd.hip_renderer import VolumeRenderer, _lib
 

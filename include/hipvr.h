@@ -54,6 +54,7 @@ extern "C" {
 
 typedef struct hvr_ctx hvr_ctx;
 typedef struct hvr_volume hvr_volume;
+typedef struct hvr_batch hvr_batch;
 
 /* Uniform block of one frame.  Replaces the shader uniforms of
  * volume.frag.glsl:3-19 set by renderer.py:99-105, 136-141, 163-172,
@@ -207,6 +208,47 @@ int hvr_skip_info(hvr_ctx *ctx, unsigned long long *clear_bricks, unsigned long 
  * used by volumes created afterwards: 2, 3 or 4 (4^3, 8^3 or 16^3 bricks,
  * each stored with a one-texel apron; default 3). */
 int hvr_set_brick_log2(hvr_ctx *ctx, int log2_edge);
+
+/* ---- batches (no counterpart in the reference: PyVR renders one frame per
+ * ModernGLVolumeRenderer.render() call, renderer.py:209-219).  A batch
+ * holds many render requests, each with its own volume, image size,
+ * camera/light/step parameters and LUT, and renders them all with one
+ * persistent launch per volume layout, captured once in a hipGraph and
+ * replayed.  Each request's image is the one hvr_render gives for the same
+ * bindings, LUT and parameters (byte for byte, sample counts included).
+ *
+ * A request reads its scalar field from an uploaded volume of the batch's
+ * context with brick edge 8 or 16, below 4 GiB, and its normals from the
+ * same volume (HVR_ROLE_NORMAL), its scalar (HVR_ROLE_SCALAR: (rho, 0, 0)),
+ * or none (n = NULL); other bindings and LUTs of more than 2048 entries
+ * need the single-request path.  Volumes must outlive the requests that
+ * reference them (until hvr_batch_clear or hvr_batch_destroy).  A batch
+ * owns its stream and buffers: several batches of one context may be in
+ * flight at once; one batch is used by one host thread at a time. */
+#define HVR_TILE_8X8 0   /* ray tile of a wavefront: 8 x 8 pixels (block 16 x 16) */
+#define HVR_TILE_16X4 1  /* 16 x 4 pixels (block 32 x 8) */
+int hvr_batch_create(hvr_ctx *ctx, hvr_batch **out);
+int hvr_batch_destroy(hvr_batch *batch);
+int hvr_batch_clear(hvr_batch *batch);
+int hvr_batch_set_tile_shape(hvr_batch *batch, int shape);
+/* Add a request; lut_rgba holds lut_n RGBA float entries (lut_n = 0: no
+ * LUT bound, reads (0, 0, 0, 1)); *index receives the request's index. */
+int hvr_batch_add(hvr_batch *batch, hvr_volume *scalar_vol, int scalar_role, hvr_volume *normal_vol,
+                  int normal_role, const float *lut_rgba, int lut_n, const hvr_params *params, int width,
+                  int height, int *index);
+/* Enqueue the whole batch on the batch's stream (count != 0: also record
+ * per-pixel sample counts); the images are copied to pinned host memory.
+ * hvr_batch_run waits for them. */
+int hvr_batch_run_async(hvr_batch *batch, int count);
+int hvr_batch_wait(hvr_batch *batch);
+int hvr_batch_run(hvr_batch *batch, int count);
+/* After the run completed: request `index`'s width*height*4 RGBA8 bytes
+ * (bottom row first) / its per-pixel sample counts, valid until the next
+ * run of the batch. */
+int hvr_batch_output(hvr_batch *batch, int index, const uint8_t **rgba);
+int hvr_batch_counts(hvr_batch *batch, int index, const uint32_t **counts);
+/* Device time of the last run's graph (ms), without the copy back. */
+int hvr_batch_last_ms(hvr_batch *batch, float *ms);
 
 #ifdef __cplusplus
 }

@@ -11,6 +11,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -132,22 +133,21 @@ static bool inverse4(const double *m, double *inv)
 // Uniforms -> kernel constants.  The matrices arrive as numpy row-major
 // float32 bytes that GLSL reads column-major, i.e. transposed
 // (manager.py:192); they are inverted in double and rounded once to float.
-static FrameConst frame_const(const hvr_ctx *c)
+static FrameConst frame_const_of(const hvr_params &p, bool params_set, int W, int H)
 {
     FrameConst f;
     memset(&f, 0, sizeof(f));
-    const hvr_params &p = c->params;
-    f.W = c->W;
-    f.H = c->H;
-    f.fW = (float)c->W;
-    f.fH = (float)c->H;
+    f.W = W;
+    f.H = H;
+    f.fW = (float)W;
+    f.fH = (float)H;
     double pg[16], vg[16], pinv[16], vinv[16];
     for (int r = 0; r < 4; ++r)
         for (int k = 0; k < 4; ++k) {
             pg[4 * r + k] = (double)p.proj[4 * k + r];
             vg[4 * r + k] = (double)p.view[4 * k + r];
         }
-    f.ok = c->params_set && p.camera_set && inverse4(pg, pinv) && inverse4(vg, vinv);
+    f.ok = params_set && p.camera_set && inverse4(pg, pinv) && inverse4(vg, vinv);
     if (f.ok) {
         for (int k = 0; k < 8; ++k) f.prow[k] = (float)pinv[k];
         for (int r = 0; r < 3; ++r)
@@ -178,6 +178,8 @@ static FrameConst frame_const(const hvr_ctx *c)
     return f;
 }
 
+static FrameConst frame_const(const hvr_ctx *c) { return frame_const_of(c->params, c->params_set, c->W, c->H); }
+
 // Texture descriptor for a frame: texture x samples world z, y samples y,
 // z samples x (the shader's swizzle), so k0 = s0/ext_z, k1 = s1/ext_y,
 // k2 = s2/ext_x with ext the float box extent of the in-box test.
@@ -197,7 +199,8 @@ static TexDesc tex_desc(const hvr_volume *v, int role, const FrameConst &fc)
     t.nbx = v->nbx;
     t.nby = v->nby;
     t.brick_bytes = v->brick_bytes;
-    t.nbytes32 = v->bytes <= 0xffffffffull ? (uint32_t)v->bytes : 0u;
+    const uint64_t nbricks = (uint64_t)v->nbx * v->nby * v->nbz;
+    t.nbytes32 = v->bytes <= 0xffffffffull && nbricks < (1ull << 24) ? (uint32_t)v->bytes : 0u;
     t.fmt = v->fmt;
     t.role = role;
     return t;
@@ -667,6 +670,25 @@ int hvr_volume_destroy(hvr_volume *v)
     return HVR_OK;
 }
 
+// Prefix counts (n + 1 values) of LUT entries that are not exactly
+// transparent.  The march reads entry i as v[i] + a*(v[i+1] - v[i]) with a
+// in [0, 1), so entry i yields alpha 0 and a finite colour for every a (the
+// reach of a > 0 into entry i+1 is accounted for by the brick rule) only
+// when its alpha is 0 and the colours of it and of its successor, and the
+// successor's alpha, are bounded (|x| <= 1e38 also excludes NaN and
+// overflow of the difference: 0 * inf is NaN).
+static void lut_prefix_counts(const float *src, int cnt, int *nz)
+{
+    auto bounded = [](float x) { return fabsf(x) <= 1e38f; };
+    nz[0] = 0;
+    for (int i = 0; i < cnt; ++i) {
+        const float *e = src + 4 * i, *f = i + 1 < cnt ? e + 4 : e;
+        const bool clear = e[3] == 0.f && bounded(e[0]) && bounded(e[1]) && bounded(e[2]) && bounded(f[0]) &&
+                           bounded(f[1]) && bounded(f[2]) && bounded(f[3]);
+        nz[i + 1] = nz[i] + (clear ? 0 : 1);
+    }
+}
+
 int hvr_set_lut(hvr_ctx *c, const float *rgba, int n)
 {
     if (!c) return fail(HVR_EINVAL, "ctx is null");
@@ -684,21 +706,8 @@ int hvr_set_lut(hvr_ctx *c, const float *rgba, int n)
         c->lut_cap = cnt;
     }
     HIPCHK(hipMemcpy(c->d_lut, src, (size_t)cnt * sizeof(float4), hipMemcpyHostToDevice));
-    // Prefix counts of entries that are not exactly transparent.  The march
-    // reads entry i as v[i] + a*(v[i+1] - v[i]) with a in [0, 1), so entry i
-    // yields alpha 0 and a finite colour for every a (the reach of a > 0 into
-    // entry i+1 is accounted for by the brick rule) only when its alpha is 0
-    // and the colours of it and of its successor, and the successor's alpha,
-    // are bounded (|x| <= 1e38 also excludes NaN and overflow of the
-    // difference: 0 * inf is NaN).
-    auto bounded = [](float x) { return fabsf(x) <= 1e38f; };
     std::vector<int> nz((size_t)cnt + 1, 0);
-    for (int i = 0; i < cnt; ++i) {
-        const float *e = src + 4 * i, *f = i + 1 < cnt ? e + 4 : e;
-        const bool clear = e[3] == 0.f && bounded(e[0]) && bounded(e[1]) && bounded(e[2]) && bounded(f[0]) &&
-                           bounded(f[1]) && bounded(f[2]) && bounded(f[3]);
-        nz[i + 1] = nz[i] + (clear ? 0 : 1);
-    }
+    lut_prefix_counts(src, cnt, nz.data());
     (void)hipFree(c->d_nz);
     c->d_nz = nullptr;
     HIPCHK(hipMalloc(&c->d_nz, nz.size() * sizeof(int)));
@@ -708,9 +717,8 @@ int hvr_set_lut(hvr_ctx *c, const float *rgba, int n)
     return HVR_OK;
 }
 
-int hvr_set_params(hvr_ctx *c, const hvr_params *p)
+static int check_params(const hvr_params *p)
 {
-    if (!c || !p) return fail(HVR_EINVAL, "null argument");
     if (!(p->step > 0.f) || !(p->ref_step > 0.f)) return fail(HVR_EINVAL, "step sizes must be positive");
     if (p->max_steps < 0) return fail(HVR_EINVAL, "max_steps must be >= 0");
     for (int k = 0; k < 3; ++k) {
@@ -720,6 +728,13 @@ int hvr_set_params(hvr_ctx *c, const hvr_params *p)
         if (!std::isfinite(p->bmin[k]) || !std::isfinite(p->bmax[k]) || !(ext >= 1e-20f) || !std::isfinite(ext))
             return fail(HVR_EINVAL, "box bounds must be finite with bmax - bmin >= 1e-20 on every axis");
     }
+    return HVR_OK;
+}
+
+int hvr_set_params(hvr_ctx *c, const hvr_params *p)
+{
+    if (!c || !p) return fail(HVR_EINVAL, "null argument");
+    if (const int rc = check_params(p)) return rc;
     c->params = *p;
     c->params_set = true;
     return HVR_OK;
@@ -889,6 +904,418 @@ int hvr_last_kernel_ms(hvr_ctx *c, float *march_ms, float *setup_ms)
     HIPCHK(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
     if (setup_ms) *setup_ms = a;
     if (march_ms) *march_ms = b;
+    return HVR_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches: requests staged on the host, rendered by one persistent launch
+// per kernel class, captured once in a hipGraph per batch shape (classes,
+// LUT capacity, staging block) and replayed with new staged contents.
+
+struct BatchReq {
+    hvr_volume *vol;
+    int nm;                  // NM_ZERO / NM_ALIAS / NM_SAME
+    hvr_params params;
+    int W, H;
+    std::vector<float> lut;  // n x RGBA
+};
+
+struct hvr_batch {
+    hvr_ctx *ctx;
+    hipStream_t stream;
+    std::vector<BatchReq> reqs;
+    int tile_shape;                       // HVR_TILE_8X8 / HVR_TILE_16X4
+    char *h_stage, *d_stage;              // staged block (pinned host, device)
+    size_t stage_cap;
+    uchar4 *d_out;                        // output pool, one RGBA8 image per request
+    uint8_t *h_out;
+    size_t out_cap;
+    uint32_t *d_counts, *h_counts;        // sample-count pool (counting runs)
+    size_t counts_cap;                    // in pixels
+    uint8_t *d_flags;                     // transparent-brick flags, per request
+    size_t flags_cap;
+    uint32_t *d_counters;                 // tile counter per class
+    std::vector<size_t> out_off;          // per request (add order): pixel offset in the pools
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    std::string key;                      // shape the graph was captured for
+    hipEvent_t ev0, ev1;
+    bool ran, counted;
+    size_t out_pixels;
+    int ncu;
+};
+
+typedef int (*BatchLaunchFn)(const char *blk, int cls, uint32_t *ctr, int lut_cap, int ncu, hipStream_t st);
+
+template <int SF, int NM, int LB, bool COUNT, int TS>
+static int batch_launch(const char *blk, int cls, uint32_t *ctr, int lut_cap, int ncu, hipStream_t st)
+{
+    const size_t lds = 2 * (size_t)lut_cap * sizeof(float4);
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch<SF, NM, LB, COUNT, TS>, 256, lds) != hipSuccess ||
+        per < 1)
+        per = 1;
+    hipLaunchKernelGGL((k_batch<SF, NM, LB, COUNT, TS>), dim3(per * ncu), dim3(256), lds, st, blk, cls, ctr, lut_cap);
+    return hipGetLastError() == hipSuccess ? HVR_OK : HVR_EDEVICE;
+}
+
+template <int SF, int NM, int LB, bool COUNT>
+static BatchLaunchFn batch_pick_ts(int ts)
+{
+    return ts == HVR_TILE_16X4 ? batch_launch<SF, NM, LB, COUNT, 1> : batch_launch<SF, NM, LB, COUNT, 0>;
+}
+
+template <int SF, int NM, bool COUNT>
+static BatchLaunchFn batch_pick_lb(int lb, int ts)
+{
+    return lb == 4 ? batch_pick_ts<SF, NM, 4, COUNT>(ts) : batch_pick_ts<SF, NM, 3, COUNT>(ts);
+}
+
+template <bool COUNT>
+static BatchLaunchFn batch_fn_c(int fmt, int nm, int lb, int ts)
+{
+#define HVR_S(F)                                                                   \
+    case F:                                                                        \
+        if (nm == NM_ZERO) return batch_pick_lb<F, NM_ZERO, COUNT>(lb, ts);        \
+        if (nm == NM_ALIAS) return batch_pick_lb<F, NM_ALIAS, COUNT>(lb, ts);      \
+        return nullptr;
+#define HVR_C(F)                                                                   \
+    case F:                                                                        \
+        if (nm == NM_SAME) return batch_pick_lb<F, NM_SAME, COUNT>(lb, ts);        \
+        return nullptr;
+    switch (fmt) {
+        HVR_S(TF_S_F32)
+        HVR_S(TF_S_F16)
+        HVR_S(TF_S_U8)
+        HVR_C(TF_C_F32_F32)
+        HVR_C(TF_C_F16_F32)
+        HVR_C(TF_C_U8_F32)
+        HVR_C(TF_C_F32_SN8)
+        HVR_C(TF_C_F16_SN8)
+        HVR_C(TF_C_U8_SN8)
+    default: return nullptr;
+    }
+#undef HVR_S
+#undef HVR_C
+}
+
+static BatchLaunchFn batch_fn(int fmt, int nm, int lb, bool count, int ts)
+{
+    return count ? batch_fn_c<true>(fmt, nm, lb, ts) : batch_fn_c<false>(fmt, nm, lb, ts);
+}
+
+static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Grow a device (and optionally pinned host) buffer to at least `need`
+// bytes.  Returns true when it was reallocated.
+static int grow(void **d, void **h, size_t *cap, size_t need, bool *moved)
+{
+    if (need <= *cap) return HVR_OK;
+    size_t n = *cap ? *cap : 4096;
+    while (n < need) n *= 2;
+    (void)hipFree(*d);
+    *d = nullptr;
+    if (h) {
+        (void)hipHostFree(*h);
+        *h = nullptr;
+    }
+    *cap = 0;
+    HIPCHK(hipMalloc(d, n));
+    if (h) HIPCHK(hipHostMalloc(h, n, hipHostMallocDefault));
+    *cap = n;
+    *moved = true;
+    return HVR_OK;
+}
+
+static void batch_drop_graph(hvr_batch *b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->key.clear();
+}
+
+extern "C" {
+
+int hvr_batch_create(hvr_ctx *c, hvr_batch **out)
+{
+    if (!c || !out) return fail(HVR_EINVAL, "null argument");
+    *out = nullptr;
+    HIPCHK(hipSetDevice(c->device));
+    hvr_batch *b = new hvr_batch();
+    b->ctx = c;
+    b->tile_shape = HVR_TILE_8X8;
+    b->exec = nullptr;
+    b->graph = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    if (e == hipSuccess) e = hipMalloc(&b->d_counters, kBatchMaxClasses * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, c->device);
+    if (e != hipSuccess) {
+        hvr_batch_destroy(b);
+        return fail_hip(e, "hvr_batch_create");
+    }
+    *out = b;
+    return HVR_OK;
+}
+
+int hvr_batch_destroy(hvr_batch *b)
+{
+    if (!b) return HVR_OK;
+    (void)hipSetDevice(b->ctx->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    batch_drop_graph(b);
+    (void)hipFree(b->d_stage);
+    (void)hipFree(b->d_out);
+    (void)hipFree(b->d_counts);
+    (void)hipFree(b->d_flags);
+    (void)hipFree(b->d_counters);
+    if (b->h_stage) (void)hipHostFree(b->h_stage);
+    if (b->h_out) (void)hipHostFree(b->h_out);
+    if (b->h_counts) (void)hipHostFree(b->h_counts);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return HVR_OK;
+}
+
+int hvr_batch_clear(hvr_batch *b)
+{
+    if (!b) return fail(HVR_EINVAL, "batch is null");
+    b->reqs.clear();
+    b->ran = b->counted = false;
+    return HVR_OK;
+}
+
+int hvr_batch_set_tile_shape(hvr_batch *b, int shape)
+{
+    if (!b) return fail(HVR_EINVAL, "batch is null");
+    if (shape != HVR_TILE_8X8 && shape != HVR_TILE_16X4) return fail(HVR_EINVAL, "unknown tile shape");
+    b->tile_shape = shape;
+    return HVR_OK;
+}
+
+int hvr_batch_add(hvr_batch *b, hvr_volume *s, int srole, hvr_volume *n, int nrole, const float *lut_rgba,
+                  int lut_n, const hvr_params *p, int width, int height, int *index)
+{
+    if (!b || !p) return fail(HVR_EINVAL, "null argument");
+    if (!s || s->ctx != b->ctx || (n && n->ctx != b->ctx))
+        return fail(HVR_EINVAL, "a batch request needs a scalar volume of the batch's context");
+    if (srole != HVR_ROLE_SCALAR || !fmt_has_scalar(s->fmt) || !s->scalar_loaded || !s->d_range)
+        return fail(HVR_EINVAL, "the scalar sampler must read an uploaded scalar field");
+    int nm;
+    if (!n) nm = NM_ZERO;
+    else if (n == s && nrole == HVR_ROLE_SCALAR) nm = NM_ALIAS;
+    else if (n == s && nrole == HVR_ROLE_NORMAL && fmt_has_normal(s->fmt)) nm = NM_SAME;
+    else return fail(HVR_EINVAL, "a batch request reads its normals from its scalar volume, or none");
+    if (!batch_fn(s->fmt, nm, s->lb, false, HVR_TILE_8X8))
+        return fail(HVR_EINVAL, "this volume layout and normal binding need the single-request path");
+    if (s->bytes > 0xffffffffull || (uint64_t)s->nbx * s->nby * s->nbz >= (1ull << 24))
+        return fail(HVR_EINVAL, "batched volumes must be below 4 GiB (and 2^24 bricks)");
+    if (lut_n < 0 || lut_n > kLutLdsMax || (lut_n > 0 && !lut_rgba))
+        return fail(HVR_EINVAL, "batched LUTs hold 0 to 2048 entries");
+    if (width <= 0 || height <= 0 || (long long)width * height > (1ll << 28))
+        return fail(HVR_EINVAL, "width and height must be positive (at most 2^28 pixels)");
+    if (const int rc = check_params(p)) return rc;
+    BatchReq r;
+    r.vol = s;
+    r.nm = nm;
+    r.params = *p;
+    r.W = width;
+    r.H = height;
+    if (lut_n > 0) r.lut.assign(lut_rgba, lut_rgba + 4 * (size_t)lut_n);
+    else r.lut = {0.f, 0.f, 0.f, 1.f};  // unbound LUT texture
+    b->reqs.push_back(std::move(r));
+    b->ran = b->counted = false;
+    if (index) *index = (int)b->reqs.size() - 1;
+    return HVR_OK;
+}
+
+int hvr_batch_run_async(hvr_batch *b, int count)
+{
+    if (!b) return fail(HVR_EINVAL, "batch is null");
+    hvr_ctx *c = b->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    // the previous run's copies read and write the pinned buffers
+    HIPCHK(hipStreamSynchronize(b->stream));
+    const int nreq = (int)b->reqs.size();
+    b->ran = b->counted = false;
+    if (nreq == 0) return HVR_OK;
+    // classes: one kernel instance each, in order of first appearance
+    struct Cls {
+        int fmt, nm, lb;
+        std::vector<int> members;
+    };
+    std::vector<Cls> cls;
+    int lut_cap = 1;
+    for (int i = 0; i < nreq; ++i) {
+        const BatchReq &r = b->reqs[i];
+        int k = 0;
+        while (k < (int)cls.size() && !(cls[k].fmt == r.vol->fmt && cls[k].nm == r.nm && cls[k].lb == r.vol->lb)) ++k;
+        if (k == (int)cls.size()) {
+            if (k == kBatchMaxClasses) return fail(HVR_EINVAL, "a batch mixes more than 8 volume layouts");
+            cls.push_back({r.vol->fmt, r.nm, r.vol->lb, {}});
+        }
+        cls[k].members.push_back(i);
+        lut_cap = std::max(lut_cap, (int)(r.lut.size() / 4));
+    }
+    // LDS capacity in steps of 256 entries, so that graphs are reused
+    lut_cap = (int)align_up((size_t)lut_cap, 256);
+    // costliest requests first inside each class (tail balance)
+    for (Cls &k : cls)
+        std::stable_sort(k.members.begin(), k.members.end(), [&](int a, int z) {
+            return (long long)b->reqs[a].W * b->reqs[a].H > (long long)b->reqs[z].W * b->reqs[z].H;
+        });
+    // sizes
+    const int bx = b->tile_shape == HVR_TILE_16X4 ? 32 : 16, by = b->tile_shape == HVR_TILE_16X4 ? 8 : 16;
+    size_t desc_off = align_up(sizeof(BatchHeader), 256);
+    size_t lut_off = align_up(desc_off + (size_t)nreq * sizeof(ReqDesc), 256);
+    size_t lut_bytes = 0, nz_bytes = 0, pixels = 0, flag_bytes = 0;
+    b->out_off.assign(nreq, 0);
+    for (int i = 0; i < nreq; ++i) {
+        const BatchReq &r = b->reqs[i];
+        lut_bytes += r.lut.size() * sizeof(float);
+        nz_bytes += (r.lut.size() / 4 + 1) * sizeof(int);
+        b->out_off[i] = pixels;
+        pixels += align_up((size_t)r.W * r.H, 64);
+        flag_bytes += align_up((size_t)r.vol->nbx * r.vol->nby * r.vol->nbz, 256);
+    }
+    const size_t nz_off = align_up(lut_off + lut_bytes, 256);
+    const size_t stage_need = align_up(nz_off + nz_bytes, 256);
+    bool moved = false;
+    if (const int rc = grow((void **)&b->d_stage, (void **)&b->h_stage, &b->stage_cap, stage_need, &moved)) return rc;
+    bool moved_out = false;
+    if (const int rc = grow((void **)&b->d_out, (void **)&b->h_out, &b->out_cap, pixels * 4, &moved_out)) return rc;
+    if (const int rc = grow((void **)&b->d_flags, nullptr, &b->flags_cap, flag_bytes, &moved_out)) return rc;
+    if (count) {
+        size_t cb = b->counts_cap * 4;
+        if (const int rc = grow((void **)&b->d_counts, (void **)&b->h_counts, &cb, pixels * 4, &moved_out)) return rc;
+        b->counts_cap = cb / 4;
+    }
+    if (moved) batch_drop_graph(b);
+    // stage: header, descriptors (class by class), LUTs, prefix counts
+    memset(b->h_stage, 0, desc_off);
+    BatchHeader *hdr = (BatchHeader *)b->h_stage;
+    hdr->nclass = (int)cls.size();
+    hdr->nreq = nreq;
+    hdr->all_offset = (uint32_t)desc_off;
+    ReqDesc *descs = (ReqDesc *)(b->h_stage + desc_off);
+    size_t lut_at = lut_off, nz_at = nz_off, flag_at = 0;
+    int slot = 0;
+    for (size_t k = 0; k < cls.size(); ++k) {
+        hdr->cls[k].desc_offset = (uint32_t)(desc_off + (size_t)slot * sizeof(ReqDesc));
+        hdr->cls[k].nreq = (int)cls[k].members.size();
+        int tiles = 0;
+        for (int i : cls[k].members) {
+            const BatchReq &r = b->reqs[i];
+            ReqDesc &d = descs[slot++];
+            memset(&d, 0, sizeof(d));
+            d.fc = frame_const_of(r.params, true, r.W, r.H);
+            d.ts = tex_desc(r.vol, HVR_ROLE_SCALAR, d.fc);
+            const int n = (int)(r.lut.size() / 4);
+            memcpy(b->h_stage + lut_at, r.lut.data(), r.lut.size() * sizeof(float));
+            lut_prefix_counts(r.lut.data(), n, (int *)(b->h_stage + nz_at));
+            d.lut = (const float4 *)(b->d_stage + lut_at);
+            d.nz = (const int *)(b->d_stage + nz_at);
+            lut_at += r.lut.size() * sizeof(float);
+            nz_at += (size_t)(n + 1) * sizeof(int);
+            d.lut_n = n;
+            d.range = r.vol->d_range;
+            d.nbricks = r.vol->nbx * r.vol->nby * r.vol->nbz;
+            d.flags = b->d_flags + flag_at;
+            flag_at += align_up(d.nbricks, 256);
+            d.out = b->d_out + b->out_off[i];
+            d.counts = count ? b->d_counts + b->out_off[i] : nullptr;
+            d.ntx = (r.W + bx - 1) / bx;
+            tiles += d.ntx * ((r.H + by - 1) / by);
+            d.tile_end = tiles;
+        }
+    }
+    // the graph for this shape
+    std::string key = std::to_string(b->stage_cap) + "/" + std::to_string(lut_cap) + "/" +
+                      std::to_string(count) + "/" + std::to_string(b->tile_shape);
+    for (const Cls &k : cls)
+        key += "/" + std::to_string(k.fmt) + "." + std::to_string(k.nm) + "." + std::to_string(k.lb);
+    if (key != b->key || !b->exec) {
+        batch_drop_graph(b);
+        HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        int rc = HVR_OK;
+        if (hipMemcpyAsync(b->d_stage, b->h_stage, b->stage_cap, hipMemcpyHostToDevice, b->stream) != hipSuccess)
+            rc = HVR_EDEVICE;
+        if (rc == HVR_OK) {
+            // counters reset by a kernel (a graph replays kernels reliably)
+            hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, b->stream, b->d_counters, kBatchMaxClasses, 0u);
+            hipLaunchKernelGGL(k_batch_flags, dim3(1024), dim3(256), 0, b->stream, (const char *)b->d_stage);
+            if (hipGetLastError() != hipSuccess) rc = HVR_EDEVICE;
+        }
+        for (size_t k = 0; k < cls.size() && rc == HVR_OK; ++k)
+            rc = batch_fn(cls[k].fmt, cls[k].nm, cls[k].lb, count != 0, b->tile_shape)(
+                b->d_stage, (int)k, b->d_counters + k, lut_cap, b->ncu, b->stream);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (rc != HVR_OK || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return fail(HVR_EDEVICE, "capturing the batch graph failed");
+        }
+        b->graph = g;
+        HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        b->key = key;
+    }
+    HIPCHK(hipEventRecord(b->ev0, b->stream));
+    HIPCHK(hipGraphLaunch(b->exec, b->stream));
+    HIPCHK(hipEventRecord(b->ev1, b->stream));
+    HIPCHK(hipMemcpyAsync(b->h_out, b->d_out, pixels * 4, hipMemcpyDeviceToHost, b->stream));
+    if (count) HIPCHK(hipMemcpyAsync(b->h_counts, b->d_counts, pixels * 4, hipMemcpyDeviceToHost, b->stream));
+    b->out_pixels = pixels;
+    b->ran = true;
+    b->counted = count != 0;
+    return HVR_OK;
+}
+
+int hvr_batch_wait(hvr_batch *b)
+{
+    if (!b) return fail(HVR_EINVAL, "batch is null");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return HVR_OK;
+}
+
+int hvr_batch_run(hvr_batch *b, int count)
+{
+    if (const int rc = hvr_batch_run_async(b, count)) return rc;
+    return hvr_batch_wait(b);
+}
+
+int hvr_batch_output(hvr_batch *b, int index, const uint8_t **rgba)
+{
+    if (!b || !rgba) return fail(HVR_EINVAL, "null argument");
+    if (!b->ran) return fail(HVR_ESTATE, "the batch has not been run since its last change");
+    if (index < 0 || index >= (int)b->reqs.size()) return fail(HVR_EINVAL, "request index out of range");
+    *rgba = b->h_out + 4 * b->out_off[index];
+    return HVR_OK;
+}
+
+int hvr_batch_counts(hvr_batch *b, int index, const uint32_t **counts)
+{
+    if (!b || !counts) return fail(HVR_EINVAL, "null argument");
+    if (!b->ran || !b->counted) return fail(HVR_ESTATE, "the batch has not been run with counting");
+    if (index < 0 || index >= (int)b->reqs.size()) return fail(HVR_EINVAL, "request index out of range");
+    *counts = b->h_counts + b->out_off[index];
+    return HVR_OK;
+}
+
+int hvr_batch_last_ms(hvr_batch *b, float *ms)
+{
+    if (!b || !ms) return fail(HVR_EINVAL, "null argument");
+    if (!b->ran) return fail(HVR_ESTATE, "the batch has not been run");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    HIPCHK(hipEventSynchronize(b->ev1));
+    HIPCHK(hipEventElapsedTime(ms, b->ev0, b->ev1));
     return HVR_OK;
 }
 

@@ -81,12 +81,44 @@ struct TexDesc {
     int lb;                // log2 of the brick interior edge
     uint32_t nbx, nby;     // bricks along x and y
     uint32_t brick_bytes;  // E^3 * texel bytes
-    uint32_t nbytes32;     // allocation bytes when below 4 GiB (buffer-load path), else 0
+    uint32_t nbytes32;     // allocation bytes when below 4 GiB with fewer than 2^24
+                           // bricks (buffer-load path), else 0
     int fmt;
     int role;              // field a sampler reads: 0 scalar, 1 normal
 };
 
 __host__ __device__ constexpr uint32_t bricks_along(int s, int lb) { return ((uint32_t)s >> lb) + 1u; }
+
+// Loads and stores through pointers the compiler cannot place (fetched from
+// descriptors in memory): they address device global memory, so issue
+// global (not flat) instructions.
+template <typename T>
+__device__ __forceinline__ T ld_global(const T *p)
+{
+    return *(const __attribute__((address_space(1))) T *)p;
+}
+__device__ __forceinline__ float4 ld_global(const float4 *p)
+{
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    const v4 v = *(const __attribute__((address_space(1))) v4 *)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float2 ld_global(const float2 *p)
+{
+    typedef float v2 __attribute__((ext_vector_type(2)));
+    const v2 v = *(const __attribute__((address_space(1))) v2 *)p;
+    return make_float2(v.x, v.y);
+}
+template <typename T>
+__device__ __forceinline__ void st_global(T *p, const T &v)
+{
+    *(__attribute__((address_space(1))) T *)p = v;
+}
+__device__ __forceinline__ void st_global(uchar4 *p, const uchar4 &v)
+{
+    const uint32_t w = (uint32_t)v.x | ((uint32_t)v.y << 8) | ((uint32_t)v.z << 16) | ((uint32_t)v.w << 24);
+    *(__attribute__((address_space(1))) uint32_t *)p = w;
+}
 
 __device__ __forceinline__ float sn8f(uint32_t w, int sh) { return (float)(int8_t)((w >> sh) & 0xffu); }
 
@@ -203,12 +235,18 @@ __device__ __forceinline__ Footprint footprint(const TexDesc &t, float w0, float
     f.w = w;
     const uint32_t X = (uint32_t)(int)fu, Y = (uint32_t)(int)fv, Z = (uint32_t)(int)fw;
     constexpr uint32_t M = (1u << LB) - 1u, E = (1u << LB) + 1u;
-    const uint32_t brick = ((Z >> LB) * t.nby + (Y >> LB)) * t.nbx + (X >> LB);
     const uint32_t local = ((Z & M) * E + (Y & M)) * E + (X & M);
+    uint32_t brick;
     if constexpr (BUF) {
-        f.off = brick * t.brick_bytes + local * (uint32_t)T;   // < nbytes32 < 2^32
+        // The buffer path is only used with nbricks < 2^24 (and the
+        // allocation below 4 GiB): every partial brick index and
+        // brick_bytes (<= 17^3 * 16) fit 24 bits and the byte offset 32, so
+        // single-pass 24-bit multiplies are exact.
+        brick = __umul24(__umul24(Z >> LB, t.nby) + (Y >> LB), t.nbx) + (X >> LB);
+        f.off = __umul24(brick, t.brick_bytes) + local * (uint32_t)T;   // < nbytes32 < 2^32
         f.p = nullptr;
     } else {
+        brick = ((Z >> LB) * t.nby + (Y >> LB)) * t.nbx + (X >> LB);
         f.p = (const char *)t.base + (uint64_t)brick * t.brick_bytes + (uint64_t)(local * (uint32_t)T);
         f.off = 0;
     }
@@ -448,14 +486,12 @@ struct __align__(16) RayRec {
     int pad;
 };
 
-// K1 -- ray setup (volume.vert.glsl:8; volume.frag.glsl:47-68, 71-82).
+// Ray setup (volume.vert.glsl:8; volume.frag.glsl:47-68, 71-82); K1 runs it
+// per pixel into a buffer, the batch kernel inline.
 // Pixel (px, py), py = 0 the bottom row.  Positions use separately rounded
 // products and sums (no fused multiply-add), as the shader states them.
-__global__ void __launch_bounds__(256) k_ray_setup(const FrameConst fc, RayRec *__restrict__ rays)
+__device__ __forceinline__ RayRec ray_setup(const FrameConst &fc, int px, int py)
 {
-    const int idx = blockIdx.x * 256 + threadIdx.x;
-    if (idx >= fc.W * fc.H) return;
-    const int px = idx % fc.W, py = idx / fc.W;
     RayRec r;
     r.p0x = r.p0y = r.p0z = 0.f;
     r.dsx = r.dsy = r.dsz = 0.f;
@@ -501,7 +537,14 @@ __global__ void __launch_bounds__(256) k_ray_setup(const FrameConst fc, RayRec *
             r.nlim = span < (float)fc.max_steps ? (int)span : fc.max_steps;
         }
     }
-    rays[idx] = r;
+    return r;
+}
+
+__global__ void __launch_bounds__(256) k_ray_setup(const FrameConst fc, RayRec *__restrict__ rays)
+{
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= fc.W * fc.H) return;
+    rays[idx] = ray_setup(fc, idx % fc.W, idx / fc.W);
 }
 
 // Stored RGBA8 (manager.py:29-31, 212-220): the source colour is clamped
@@ -614,41 +657,16 @@ __device__ __forceinline__ bool inbox(const FrameConst &fc, float wx, float wy, 
     return (wz >= 0.f) & (wz <= fc.ext[2]) & (wy >= 0.f) & (wy <= fc.ext[1]) & (wx >= 0.f) & (wx <= fc.ext[0]);
 }
 
+// The march of one ray (volume.frag.glsl:84-121) from its setup record:
+// accumulated colour (C0, C1, C2), alpha A, and the number of composited
+// samples ns (COUNT).  s_val / s_dlt: the LUT and its differences in LDS.
 template <int SF, int NM, int LB, bool COUNT, bool SKIP, bool BUF>
-__global__ void __launch_bounds__(256)
-k_march(const FrameConst fc, const TexDesc ts, const TexDesc tn, const float4 *__restrict__ lut_g,
-        int lut_n, const RayRec *__restrict__ rays, uchar4 *__restrict__ out,
-        uint32_t *__restrict__ counts, const uint8_t *__restrict__ skip, int ntx, int nblocks)
+__device__ __forceinline__ void march_ray(const FrameConst &fc, const TexDesc &ts, const TexDesc &tn,
+                                          const float4 *__restrict__ lut_g, int lut_n, const float4 *s_val,
+                                          const float4 *s_dlt, const RayRec &r, const uint8_t *__restrict__ skip,
+                                          float &C0, float &C1, float &C2, float &A, uint32_t &ns)
 {
-    extern __shared__ float4 s_lut[];
-    float4 *s_val = s_lut, *s_dlt = s_lut + lut_n;
-    if constexpr (NM != NM_ANY) {
-        for (int i = threadIdx.x; i < lut_n; i += 256) {
-            const float4 v = lut_g[i];
-            float4 d = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (i + 1 < lut_n) {
-                const float4 n = lut_g[i + 1];
-                d = make_float4(n.x - v.x, n.y - v.y, n.z - v.z, n.w - v.w);
-            }
-            s_val[i] = v;
-            s_dlt[i] = d;
-        }
-        __syncthreads();
-    }
-    const int b = blockIdx.x;
-    const int per = nblocks >> 3, full = per << 3;
-    const int logical = b < full ? (b & 7) * per + (b >> 3) : b;
-    const int tx = logical % ntx, ty = logical / ntx;
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int px = tx * 16 + (wv & 1) * 8 + (lane & 7);
-    const int py = ty * 16 + (wv >> 1) * 8 + (lane >> 3);
-    if (px >= fc.W || py >= fc.H) return;
-    const int pix = py * fc.W + px;
-    const RayRec r = rays[pix];
-
     float x = r.p0x, y = r.p0y, z = r.p0z;
-    float C0 = 0.f, C1 = 0.f, C2 = 0.f, A = 0.f;
-    uint32_t ns = 0;
     const float fn = (float)lut_n, fn1 = (float)(lut_n - 1);
     constexpr float u8k = fmt_u8_scalar(SF) ? 1.0f / 255.0f : 1.0f;
     constexpr int T = texel_bytes(SF);
@@ -722,7 +740,7 @@ k_march(const FrameConst fc, const TexDesc ts, const TexDesc tn, const float4 *_
             float wx = x - fc.bmin[0], wy = y - fc.bmin[1], wz = z - fc.bmin[2];
             bool in = i < nlim && inbox(fc, wx, wy, wz);
             Footprint f = footprint<T, LB, BUF>(ts, in ? wz : 0.f, in ? wy : 0.f, in ? wx : 0.f);
-            bool clear = in && skip[f.brick] != 0;
+            bool clear = in && ld_global(skip + f.brick) != 0;
             while (in && A < fc.tau) {
                 if (!clear) shade(f);
                 ++i;
@@ -735,10 +753,54 @@ k_march(const FrameConst fc, const TexDesc ts, const TexDesc tn, const float4 *_
                 wz = z - fc.bmin[2];
                 in = i < nlim && inbox(fc, wx, wy, wz);
                 f = footprint<T, LB, BUF>(ts, in ? wz : 0.f, in ? wy : 0.f, in ? wx : 0.f);
-                clear = in && skip[f.brick] != 0;
+                clear = in && ld_global(skip + f.brick) != 0;
             }
         }
     }
+}
+
+// Load the LUT of n entries and the differences to the next entry (0 after
+// the last) into LDS.
+__device__ __forceinline__ void load_lut_lds(const float4 *__restrict__ lut_g, int n, float4 *s_val, float4 *s_dlt)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const float4 v = ld_global(lut_g + i);
+        float4 d = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (i + 1 < n) {
+            const float4 nx = ld_global(lut_g + i + 1);
+            d = make_float4(nx.x - v.x, nx.y - v.y, nx.z - v.z, nx.w - v.w);
+        }
+        s_val[i] = v;
+        s_dlt[i] = d;
+    }
+}
+
+template <int SF, int NM, int LB, bool COUNT, bool SKIP, bool BUF>
+__global__ void __launch_bounds__(256)
+k_march(const FrameConst fc, const TexDesc ts, const TexDesc tn, const float4 *__restrict__ lut_g,
+        int lut_n, const RayRec *__restrict__ rays, uchar4 *__restrict__ out,
+        uint32_t *__restrict__ counts, const uint8_t *__restrict__ skip, int ntx, int nblocks)
+{
+    extern __shared__ float4 s_lut[];
+    float4 *s_val = s_lut, *s_dlt = s_lut + lut_n;
+    if constexpr (NM != NM_ANY) {
+        load_lut_lds(lut_g, lut_n, s_val, s_dlt);
+        __syncthreads();
+    }
+    const int b = blockIdx.x;
+    const int per = nblocks >> 3, full = per << 3;
+    const int logical = b < full ? (b & 7) * per + (b >> 3) : b;
+    const int tx = logical % ntx, ty = logical / ntx;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int px = tx * 16 + (wv & 1) * 8 + (lane & 7);
+    const int py = ty * 16 + (wv >> 1) * 8 + (lane >> 3);
+    if (px >= fc.W || py >= fc.H) return;
+    const int pix = py * fc.W + px;
+    const RayRec r = rays[pix];
+
+    float C0 = 0.f, C1 = 0.f, C2 = 0.f, A = 0.f;
+    uint32_t ns = 0;
+    march_ray<SF, NM, LB, COUNT, SKIP, BUF>(fc, ts, tn, lut_g, lut_n, s_val, s_dlt, r, skip, C0, C1, C2, A, ns);
     out[pix] = blend_store(fc, C0, C1, C2, A);
     if constexpr (COUNT) counts[pix] = ns;
 }
@@ -782,6 +844,18 @@ k_brick_range(const char *__restrict__ texels, uint32_t brick_bytes, uint32_t nb
 // is the prefix count (N + 1 values) of entries that are not exactly
 // transparent (alpha != 0, or a non-finite colour).  `n_clear` counts the
 // transparent bricks (one atomic per wave).
+__device__ __forceinline__ bool brick_clear(float2 r, const int *__restrict__ nz, int n)
+{
+    const float fn = (float)n, fn1 = (float)(n - 1);
+    // an interpolated density is off by a few ulps of the largest tap
+    const float m = fmaxf(fabsf(r.x), fabsf(r.y)) * fn * 1e-5f + 1e-3f;
+    const float ulo = r.x * fn - 0.5f - m;
+    const float uhi = r.y * fn - 0.5f + m;
+    const int i0 = (int)floorf(fminf(fmaxf(ulo, 0.f), fn1));
+    const int i1 = (int)ceilf(fminf(fmaxf(uhi, 0.f), fn1));
+    return nz[i1 + 1] - nz[i0] == 0;
+}
+
 __global__ void __launch_bounds__(256)
 k_brick_flags(const float2 *__restrict__ range, uint32_t nbricks, const int *__restrict__ nz, int n,
               uint8_t *__restrict__ flags, uint32_t *__restrict__ n_clear)
@@ -789,15 +863,7 @@ k_brick_flags(const float2 *__restrict__ range, uint32_t nbricks, const int *__r
     const uint32_t brick = blockIdx.x * 256 + threadIdx.x;
     bool clear = false;
     if (brick < nbricks) {
-        const float2 r = range[brick];
-        const float fn = (float)n, fn1 = (float)(n - 1);
-        // an interpolated density is off by a few ulps of the largest tap
-        const float m = fmaxf(fabsf(r.x), fabsf(r.y)) * fn * 1e-5f + 1e-3f;
-        const float ulo = r.x * fn - 0.5f - m;
-        const float uhi = r.y * fn - 0.5f + m;
-        const int i0 = (int)floorf(fminf(fmaxf(ulo, 0.f), fn1));
-        const int i1 = (int)ceilf(fminf(fmaxf(uhi, 0.f), fn1));
-        clear = nz[i1 + 1] - nz[i0] == 0;
+        clear = brick_clear(range[brick], nz, n);
         flags[brick] = clear ? 1 : 0;
     }
     const uint64_t m = __ballot(clear);
@@ -907,6 +973,139 @@ k_upload(char *__restrict__ dst, int s0, int s1, int s2, int lb, uint32_t nbx, u
                 const uint32_t local = (lz[k] * E + ly[j]) * E + lx[i];
                 store_part<F>(dst + (uint64_t)brick * brick_bytes + (uint64_t)local * T, part, src, l);
             }
+}
+
+// ---------------------------------------------------------------------------
+// Batched rendering: many requests (own volume, image size, camera, LUT) in
+// one persistent launch per kernel instance.  The host groups requests by
+// instance (storage format, normal mode, brick size) into classes and
+// stages, for each class, a descriptor array ordered by decreasing cost;
+// tiles of 256 pixels are numbered through the class's requests in that
+// order.  Blocks take the next tile from an atomic counter until none is
+// left, load the request's LUT into LDS when the request changes, and set
+// up and march each pixel's ray exactly as the single-request kernels do.
+
+struct ReqDesc {
+    FrameConst fc;
+    TexDesc ts;                    // scalar volume (also the normals: NM_SAME / NM_ALIAS)
+    const float4 *lut;             // lut_n RGBA entries
+    const int *nz;                 // lut_n + 1 prefix counts of non-transparent entries
+    const float2 *range;           // per-brick density range of the volume
+    uint8_t *flags;                // per-brick transparent flags for this LUT
+    uchar4 *out;                   // W*H RGBA8, bottom row first
+    uint32_t *counts;              // W*H composited-sample counts (counting runs)
+    uint32_t nbricks;
+    int lut_n;
+    int ntx;                       // tiles along x
+    int tile_end;                  // one past this request's last tile in its class
+};
+
+// Per class: where its descriptors are in the staged block, and how many.
+struct BatchClass {
+    uint32_t desc_offset;          // bytes from the block start
+    int nreq;
+};
+
+static constexpr int kBatchMaxClasses = 8;
+
+// The staged block starts with this header, then the descriptors, LUTs and
+// prefix counts.
+struct BatchHeader {
+    BatchClass cls[kBatchMaxClasses];
+    int nclass;
+    int nreq;                      // all requests (descriptors are contiguous)
+    uint32_t all_offset;           // first descriptor of the first class
+    int pad;
+};
+
+// Ray-tile shapes: a wave covers WX x WY pixels, a block 2 x 2 waves.
+template <int TS>
+struct TileShape;
+template <>
+struct TileShape<0> {
+    static constexpr int WX = 8, WY = 8;
+};
+template <>
+struct TileShape<1> {
+    static constexpr int WX = 16, WY = 4;
+};
+
+template <int SF, int NM, int LB, bool COUNT, int TS>
+__global__ void __launch_bounds__(256)
+k_batch(const char *__restrict__ block, int cls, uint32_t *__restrict__ counter, int lut_cap)
+{
+    extern __shared__ float4 s_lut[];
+    __shared__ int s_item[2];
+    constexpr int WX = TileShape<TS>::WX, WY = TileShape<TS>::WY, BX = 2 * WX, BY = 2 * WY;
+    const BatchHeader *hdr = (const BatchHeader *)block;
+    const int nreq = hdr->cls[cls].nreq;
+    if (nreq <= 0) return;
+    const ReqDesc *__restrict__ reqs = (const ReqDesc *)(block + hdr->cls[cls].desc_offset);
+    const int total = reqs[nreq - 1].tile_end;
+    float4 *s_val = s_lut, *s_dlt = s_lut + lut_cap;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int cur = -1;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const int t = (int)atomicAdd(counter, 1u);
+            int q = 0;
+            if (t < total) {
+                int lo = 0, hi = nreq - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (reqs[mid].tile_end > t) hi = mid;
+                    else lo = mid + 1;
+                }
+                q = lo;
+            }
+            s_item[0] = t;
+            s_item[1] = q;
+        }
+        __syncthreads();
+        const int t = __builtin_amdgcn_readfirstlane(s_item[0]);
+        const int q = __builtin_amdgcn_readfirstlane(s_item[1]);
+        if (t >= total) break;
+        const ReqDesc &R = reqs[q];
+        if (q != cur) {
+            load_lut_lds(R.lut, R.lut_n, s_val, s_dlt);
+            __syncthreads();
+            cur = q;
+        }
+        const int lt = t - (q ? reqs[q - 1].tile_end : 0);
+        const int tx = lt % R.ntx, ty = lt / R.ntx;
+        const int px = tx * BX + (wv & 1) * WX + lane % WX;
+        const int py = ty * BY + (wv >> 1) * WY + lane / WX;
+        if (px < R.fc.W && py < R.fc.H) {
+            const RayRec r = ray_setup(R.fc, px, py);
+            float C0 = 0.f, C1 = 0.f, C2 = 0.f, A = 0.f;
+            uint32_t ns = 0;
+            march_ray<SF, NM, LB, COUNT, true, true>(R.fc, R.ts, R.ts, R.lut, R.lut_n, s_val, s_dlt, r, R.flags,
+                                                     C0, C1, C2, A, ns);
+            const int pix = py * R.fc.W + px;
+            st_global(R.out + pix, blend_store(R.fc, C0, C1, C2, A));
+            if constexpr (COUNT) st_global(R.counts + pix, ns);
+        }
+        // every thread is done with this tile's LDS before the next fetch
+        __syncthreads();
+    }
+}
+
+// Transparent-brick flags of every request of a batch for its own LUT.
+__global__ void __launch_bounds__(256) k_batch_flags(const char *__restrict__ block)
+{
+    const BatchHeader *hdr = (const BatchHeader *)block;
+    const ReqDesc *__restrict__ reqs = (const ReqDesc *)(block + hdr->all_offset);
+    const uint32_t stride = gridDim.x * 256u;
+    for (int q = 0; q < hdr->nreq; ++q) {
+        const ReqDesc &R = reqs[q];
+        for (uint32_t b = blockIdx.x * 256u + threadIdx.x; b < R.nbricks; b += stride)
+            st_global(R.flags + b, (uint8_t)(brick_clear(ld_global(R.range + b), R.nz, R.lut_n) ? 1 : 0));
+    }
+}
+
+__global__ void __launch_bounds__(64) k_fill_u32(uint32_t *__restrict__ p, int n, uint32_t v)
+{
+    for (int i = threadIdx.x; i < n; i += 64) p[i] = v;
 }
 
 }  // namespace hvr

@@ -27,8 +27,13 @@ EXPORTS = (
     "hvr_sample_counts", "hvr_last_kernel_ms", "hvr_set_brick_log2",
     "hvr_render_async_to", "hvr_device_alloc", "hvr_device_free", "hvr_copy_to_host",
     "hvr_timing_begin", "hvr_timing_end", "hvr_device_copy", "hvr_set_empty_skip",
-    "hvr_skip_info",
+    "hvr_skip_info", "hvr_batch_create", "hvr_batch_destroy", "hvr_batch_clear", "hvr_batch_set_tile_shape",
+    "hvr_batch_add", "hvr_batch_run_async", "hvr_batch_wait", "hvr_batch_run", "hvr_batch_output",
+    "hvr_batch_counts", "hvr_batch_last_ms",
 )
+
+# batch ray-tile shapes (hvr_batch_set_tile_shape)
+TILE_8X8, TILE_16X4 = 0, 1
 
 # transparent-brick skipping modes (hvr_set_empty_skip)
 SKIP_OFF, SKIP_ON, SKIP_AUTO = 0, 1, 2
@@ -87,6 +92,17 @@ def _declare(L):
         "hvr_set_empty_skip": ([vp, i], i),
         "hvr_skip_info": ([vp, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_ulonglong),
                            ctypes.POINTER(i)], i),
+        "hvr_batch_create": ([vp, ctypes.POINTER(vp)], i),
+        "hvr_batch_destroy": ([vp], i),
+        "hvr_batch_clear": ([vp], i),
+        "hvr_batch_set_tile_shape": ([vp, i], i),
+        "hvr_batch_add": ([vp, vp, i, vp, i, fp, i, vp, i, i, ctypes.POINTER(i)], i),
+        "hvr_batch_run_async": ([vp, i], i),
+        "hvr_batch_wait": ([vp], i),
+        "hvr_batch_run": ([vp, i], i),
+        "hvr_batch_output": ([vp, i, ctypes.POINTER(u8p)], i),
+        "hvr_batch_counts": ([vp, i, ctypes.POINTER(ctypes.POINTER(ctypes.c_uint32))], i),
+        "hvr_batch_last_ms": ([vp, fp], i),
         "hvr_render_async_to": ([vp, vp], i),
         "hvr_device_alloc": ([vp, ctypes.c_uint64, ctypes.POINTER(vp)], i),
         "hvr_device_free": ([vp, vp], i),

@@ -11,7 +11,10 @@ Workloads (sizes as in the benchmark configurations):
   c1  256^3 8-blob field (seed 1), 1024x1024, high_quality, ET 0.95,
       6-point seeded OTF
   c2  512^3 32-blob field (seed 2), 1024x1024, high_quality, 24 cameras
-All take a scale-down (``size``, ``image``) for fast tests.
+  c3  64 batched requests (seed 3): FP16 blob volumes of 64^3, 128^3,
+      256^3 and 96x160x224, images of 256^2, 512^2, 768x432 and 1024^2,
+      cameras around the box, own OTFs and one of four colormaps
+All take a scale-down (``size``, ``image``, ``scale``) for fast tests.
 """
 
 from dataclasses import dataclass, field
@@ -139,6 +142,72 @@ def c2(size: int = 512, image: int = 1024, with_normals: bool = True) -> Scene:
         otf=seeded_otf(6, seed=2),
         config=RenderConfig.high_quality(), width=image, height=image, tau=0.95,
     )
+
+
+# ---------------------------------------------------------------------------
+# c3: batched requests
+
+C3_SHAPES = [(64, 64, 64), (128, 128, 128), (256, 256, 256), (96, 160, 224)]
+C3_IMAGES = [(256, 256), (512, 512), (768, 432), (1024, 1024)]
+C3_COLORMAPS = ["viridis", "plasma", "inferno", "magma"]
+
+
+@dataclass
+class Request:
+    """One render request: what a fresh renderer of (width, height) renders
+    after load_volume(volumes[volume]), set_camera, set_light,
+    set_transfer_functions and set_opacity_threshold(tau)."""
+    volume: int
+    camera: Camera
+    ctf: ColorTransferFunction
+    otf: OpacityTransferFunction
+    width: int
+    height: int
+    config: RenderConfig
+    light: Light
+    tau: float
+
+    def lut(self) -> np.ndarray:
+        n = max(self.ctf.lut_size, self.otf.lut_size)
+        rgba = np.empty((n, 4), np.float32)
+        rgba[:, :3] = self.ctf.to_lut(n)
+        rgba[:, 3] = self.otf.to_lut(n)
+        return rgba
+
+
+def c3_volumes(scale: int = 1, seed: int = 3, with_normals: bool = True, shapes=None):
+    """The batch's volumes: blob fields (8 blobs, seed 10*seed + k) over
+    [-1, 1]^3 with central-difference normals; sizes divided by ``scale``."""
+    vols = []
+    for k, shape in enumerate(shapes or C3_SHAPES):
+        shp = tuple(max(8, s // scale) for s in shape)
+        data = blob_field(shp, n_blobs=8, seed=10 * seed + k, noise=1e-3)
+        vols.append(Volume(data=data, normals=compute_normal_volume(data) if with_normals else None,
+                           min_bounds=-BOX, max_bounds=BOX))
+    return vols
+
+
+def c3_requests(n: int = 64, seed: int = 3, image_scale: int = 1, nvolumes: int = 4):
+    """``n`` requests drawn from ``default_rng(300 + seed)``: volume index,
+    image size, azimuth U[0, 2pi), elevation U[-pi/3, pi/3], distance
+    U[2.5, 4], OTF with 4-8 points (its own seed), colormap; balanced
+    preset, default termination alpha."""
+    rng = np.random.default_rng(300 + seed)
+    out = []
+    for _ in range(n):
+        v = int(rng.integers(nvolumes))
+        w, h = C3_IMAGES[int(rng.integers(len(C3_IMAGES)))]
+        az = float(rng.uniform(0.0, 2 * np.pi))
+        el = float(rng.uniform(-np.pi / 3, np.pi / 3))
+        d = float(rng.uniform(2.5, 4.0))
+        npts = int(rng.integers(4, 9))
+        otf_seed = int(rng.integers(1 << 20))
+        cmap = C3_COLORMAPS[int(rng.integers(len(C3_COLORMAPS)))]
+        out.append(Request(volume=v, camera=standard_camera(az, el, d),
+                           ctf=ColorTransferFunction.from_colormap(cmap), otf=seeded_otf(npts, otf_seed),
+                           width=max(8, w // image_scale), height=max(8, h // image_scale),
+                           config=RenderConfig.balanced(), light=standard_light(), tau=0.99))
+    return out
 
 
 # ---------------------------------------------------------------------------

@@ -1,0 +1,174 @@
+"""Batched rendering on the GPU (hvr_batch_*): every request's image and
+sample counts equal, byte for byte, what a fresh single-request renderer
+gives for the same volume, camera, transfer functions and settings -- for
+both ray-tile shapes, both brick sizes, alone and in the mix, on first
+launch and on graph replays with new contents."""
+
+import numpy as np
+import pytest
+
+import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def single(req, vol, sf, nf, brick_log2=3):
+    from pyvr_amd.hip_renderer import VolumeRenderer, _lib
+
+    r = VolumeRenderer(req.width, req.height, config=req.config, light=req.light,
+                       scalar_format=sf, normal_format=nf)
+    try:
+        _lib.check(_lib.lib().hvr_set_brick_log2(r.gl_manager.ctx, brick_log2), "set_brick")
+        r.load_volume(vol)
+        r.set_camera(req.camera)
+        r.set_transfer_functions(req.ctf, req.otf)
+        r.set_opacity_threshold(req.tau)
+        img = r.render()
+        cnt = r.sample_counts()
+    finally:
+        r.close()
+    return img, cnt
+
+
+def make_batch(vols, reqs, **kw):
+    from pyvr_amd.hip_renderer import BatchRenderer
+
+    b = BatchRenderer(**kw)
+    idx = [b.add(vols[q.volume], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
+           for q in reqs]
+    return b, idx
+
+
+def check_against_single(b, idx, vols, reqs, sf, nf, brick_log2=3):
+    for i, q in zip(idx, reqs):
+        img, cnt = single(q, vols[q.volume], sf, nf, brick_log2)
+        got = b.result(i)
+        assert len(got) == q.width * q.height * 4
+        if got != img:
+            a = np.frombuffer(got, np.uint8).astype(int)
+            e = np.frombuffer(img, np.uint8).astype(int)
+            raise AssertionError(f"request {i} ({q.width}x{q.height}, volume {q.volume}): "
+                                 f"{int((a != e).sum())} bytes differ, max {int(np.abs(a - e).max())}")
+        assert np.array_equal(b.counts(i), cnt), f"request {i}: sample counts differ"
+        assert cnt.sum() > 0
+
+
+@pytest.mark.parametrize("tile", ["8x8", "16x4"])
+def test_batch_equals_single_renders(hip, tile):
+    vols = scenes.c3_volumes(scale=4)
+    reqs = scenes.c3_requests(12, image_scale=4)
+    b, idx = make_batch(vols, reqs, scalar_format="f16", normal_format="sn8x3", tile_shape=tile)
+    try:
+        b.run(count=True)
+        print(f"[batch] {len(reqs)} requests, {tile}: {b.last_ms():.3f} ms", flush=True)
+        check_against_single(b, idx, vols, reqs, "f16", "sn8x3")
+    finally:
+        b.close()
+
+
+def test_batch_alone_and_in_mix(hip):
+    vols = scenes.c3_volumes(scale=4)
+    reqs = scenes.c3_requests(10, seed=5, image_scale=4)
+    b, idx = make_batch(vols, reqs)
+    try:
+        b.run()
+        mixed = b.results()
+    finally:
+        b.close()
+    for q, want in zip(reqs, mixed):
+        b1, (i1,) = make_batch(vols, [q])
+        try:
+            b1.run()
+            assert b1.result(i1) == want
+        finally:
+            b1.close()
+
+
+def test_batch_replays_and_reuse(hip):
+    vols = scenes.c3_volumes(scale=4)
+    reqs = scenes.c3_requests(8, seed=7, image_scale=4)
+    b, idx = make_batch(vols, reqs)
+    try:
+        b.run()
+        first = b.results()
+        for _ in range(3):           # graph replays of the same contents
+            b.run()
+            assert b.results() == first
+        # same shape, new cameras and transfer functions: the captured graph
+        # is replayed with the new staged contents
+        reqs2 = scenes.c3_requests(8, seed=8, image_scale=4)
+        b.clear()
+        idx2 = [b.add(vols[q.volume], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
+                for q in reqs2]
+        b.run(count=True)
+        check_against_single(b, idx2, vols, reqs2, "f16", "sn8x3")
+    finally:
+        b.close()
+
+
+@pytest.mark.parametrize("sf,nf,lb,normals", [("f32", "f32x3", 4, True), ("u8", "sn8x3", 3, True),
+                                               ("f16", "sn8x3", 3, False), ("f32", "f32x3", 3, False)])
+def test_batch_formats_bricks_and_no_normals(hip, sf, nf, lb, normals):
+    vols = scenes.c3_volumes(scale=4, with_normals=normals)
+    reqs = scenes.c3_requests(6, seed=11, image_scale=4)
+    b, idx = make_batch(vols, reqs, scalar_format=sf, normal_format=nf, brick_log2=lb)
+    try:
+        b.run(count=True)
+        check_against_single(b, idx, vols, reqs, sf, nf, lb)
+    finally:
+        b.close()
+
+
+def test_batch_pipeline_slots(hip):
+    """Two batches in flight on their own streams: each slot's results equal
+    a synchronous run of the same requests."""
+    from pyvr_amd.hip_renderer import BatchRenderer
+
+    vols = scenes.c3_volumes(scale=4)
+    ra = scenes.c3_requests(6, seed=21, image_scale=4)
+    rb = scenes.c3_requests(5, seed=22, image_scale=4)
+    want = []
+    for reqs in (ra, rb):
+        b, _ = make_batch(vols, reqs, pipeline=1)
+        try:
+            b.run()
+            want.append(b.results())
+        finally:
+            b.close()
+    b = BatchRenderer(pipeline=2)
+    try:
+        for q in ra:
+            b.add(vols[q.volume], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
+        sa = b.submit()
+        for q in rb:
+            b.add(vols[q.volume], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
+        sb = b.submit()
+        for _ in range(3):
+            b.rerun(sa)
+            b.rerun(sb)
+        assert b.results(sa) == want[0]
+        assert b.results(sb) == want[1]
+    finally:
+        b.close()
+
+
+def test_batch_rejects_unbatchable_requests(hip):
+    from pyvr_amd.hip_renderer import BatchRenderer
+    from pyvr_amd.transferfunctions import ColorTransferFunction
+
+    vols = scenes.c3_volumes(scale=8)
+    q = scenes.c3_requests(1, image_scale=8)[0]
+    b = BatchRenderer()
+    try:
+        with pytest.raises(RuntimeError):
+            b.add(vols[0], q.camera, ColorTransferFunction.from_colormap("viridis"), q.otf, 32, 32,
+                  lut_size=4096)
+        with pytest.raises(RuntimeError):      # nothing run yet
+            b.result(0)
+        b.add(vols[0], q.camera, q.ctf, q.otf, 32, 32)
+        with pytest.raises(RuntimeError):      # not run yet
+            b.result(0)
+        b.run()
+        assert len(b.result(0)) == 32 * 32 * 4
+    finally:
+        b.close()
