@@ -141,14 +141,25 @@ def bench_batch(args):
     from pyvr_amd.hip_renderer.manager import _DeviceVolume
 
     sf, nf = args.scalar_format, args.normal_format
-    lb = args.brick_log2 if args.brick_log2 >= 0 else 3
     t0 = time.perf_counter()
     vols = scenes.c3_volumes()
     reqs = scenes.c3_requests(args.requests)
     log(f"c3: {len(vols)} volumes, {len(reqs)} requests generated in {time.perf_counter() - t0:.1f}s")
     L = _lib.lib()
+    tuning = None
+    if args.tile_shape == "auto":
+        from pyvr_amd.hip_renderer.autotune import tune
 
-    b = BatchRenderer(scalar_format=sf, normal_format=nf, brick_log2=lb, tile_shape=args.tile_shape, pipeline=2)
+        t0 = time.perf_counter()
+        bricks = (args.brick_log2,) if args.brick_log2 >= 0 else (3, 4)
+        tuning = tune(vols, reqs, sf, nf, bricks=bricks)
+        lb, tile = tuning["brick_log2"], tuning["tile_shape"]
+        log(f"autotuner ({time.perf_counter() - t0:.1f}s): " +
+            ", ".join(f"lb{a}/{t} {ms:.2f} ms" for a, t, ms in tuning["table"]) + f" -> lb{lb}/{tile}")
+    else:
+        lb, tile = (args.brick_log2 if args.brick_log2 >= 0 else 3), args.tile_shape
+
+    b = BatchRenderer(scalar_format=sf, normal_format=nf, brick_log2=lb, tile_shape=tile, pipeline=2)
     for q in reqs:
         b.add(vols[q.volume], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
     b.run(count=True)
@@ -256,7 +267,10 @@ def bench_batch(args):
         "config": {"workload": f"c3: {len(reqs)} batched requests (FP16 blob volumes 64^3-256^3 and "
                                f"96x160x224, images 256^2-1024^2), one persistent graph launch per run",
                    "requests": len(reqs), "pixels_per_step": pixels, "samples_per_step": samples,
-                   "scalar_format": sf, "normal_format": nf, "brick_log2": lb, "tile_shape": args.tile_shape},
+                   "scalar_format": sf, "normal_format": nf, "brick_log2": lb, "tile_shape": tile,
+                   "autotuned": tuning is not None},
+        "autotune": None if tuning is None else [{"brick_log2": a, "tile_shape": t, "device_ms": ms}
+                                                 for a, t, ms in tuning["table"]],
         "batch_device_ms": float(np.mean(dev_ms)), "batched_one_at_a_time": value_sync,
         "plain_kernel": res["plain_kernel"], "plain_api": res["plain_api"],
         "batched_vs_plain_kernel": value / res["plain_kernel"],
@@ -272,7 +286,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c2", choices=["c0", "c1", "c2", "c3"])
     ap.add_argument("--requests", type=int, default=64, help="c3: requests per batch")
-    ap.add_argument("--tile-shape", default="8x8", choices=["8x8", "16x4"], help="c3: ray tile of a wavefront")
+    ap.add_argument("--tile-shape", default="auto", choices=["auto", "8x8", "16x4"],
+                    help="c3: ray tile of a wavefront (auto: the autotuner picks it and the brick size)")
     ap.add_argument("--size", type=int, default=0)
     ap.add_argument("--image", type=int, default=0)
     ap.add_argument("--cameras", type=int, default=24)

@@ -115,6 +115,27 @@ int hvr_volume_create(hvr_ctx *ctx, int s0, int s1, int s2, int scalar_fmt, int 
 int hvr_volume_upload_scalar(hvr_volume *vol, const float *data);
 /* Upload the normal field: s0*s1*s2*3 float32 values in numpy C order. */
 int hvr_volume_upload_normals(hvr_volume *vol, const float *data);
+/* Device-side construction (no counterpart in the reference, whose normals
+ * come from compute_normal_volume on the host, pyvr/datasets/synthetic.py:
+ * 109-122, and are uploaded by create_normal_texture, manager.py:110-135).
+ *
+ * hvr_volume_upload_scalar_normals: upload the float32 scalar field (numpy
+ * C order) once and compute the normals on the device, bit for bit as
+ * compute_normal_volume computes them from that float32 array (np.gradient,
+ * norm summed left to right, + 1e-8, division); every axis needs >= 2
+ * voxels.  The volume must hold both fields.
+ *
+ * hvr_volume_generate_blobs: fill the scalar field (and the normals, when
+ * the volume holds them) with a seeded Gaussian-blob field generated on the
+ * device: axes linspace(-1, 1, s); per blob amp * exp(-(axis - centre)^2 /
+ * (2 sigma^2)) per axis, multiplied in double; plus uniform noise in
+ * [-noise, noise] from a counter-based hash of (seed, voxel index); clipped
+ * to [0, 1].  centers: 3 * n_blobs values, n_blobs <= 32.  For large
+ * serving volumes that would take minutes to build on the host. */
+int hvr_volume_upload_scalar_normals(hvr_volume *vol, const float *data);
+int hvr_volume_generate_blobs(hvr_volume *vol, int n_blobs, const double *centers, const double *sigmas,
+                              const double *amps, double noise, unsigned long long seed);
+
 /* Device bytes held by the volume's texels. */
 int hvr_volume_bytes(const hvr_volume *vol, uint64_t *bytes);
 int hvr_volume_destroy(hvr_volume *vol);

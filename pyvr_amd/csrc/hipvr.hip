@@ -625,6 +625,105 @@ static int upload_part(hvr_volume *v, const float *data, int part)
     return HVR_OK;
 }
 
+// Scalar part (and, with `normals`, the normal part computed on the device)
+// of volume v from a device float32 field in numpy C order.
+static int volume_from_field(hvr_volume *v, const float *d_field, bool normals)
+{
+    hvr_ctx *c = v->ctx;
+    const uint64_t nvox = (uint64_t)v->s0 * v->s1 * v->s2;
+    const unsigned nblk = (unsigned)((nvox + 255) / 256);
+#define HVR_UPF(F)                                                                                           \
+    case F:                                                                                                  \
+        if (fmt_has_scalar(F))                                                                               \
+            hipLaunchKernelGGL(k_upload<F>, dim3(nblk), dim3(256), 0, c->stream, (char *)v->d_texels, v->s0,   \
+                               v->s1, v->s2, v->lb, v->nbx, v->nby, v->nbz, v->brick_bytes, d_field, 1, 0, nvox); \
+        if (normals && fmt_has_normal(F))                                                                    \
+            hipLaunchKernelGGL(k_normals_from_field<F>, dim3(nblk), dim3(256), 0, c->stream,                 \
+                               (char *)v->d_texels, v->s0, v->s1, v->s2, v->lb, v->nbx, v->nby, v->nbz,      \
+                               v->brick_bytes, d_field, nvox);                                               \
+        break;
+    switch (v->fmt) {
+        HVR_UPF(TF_S_F32)
+        HVR_UPF(TF_S_F16)
+        HVR_UPF(TF_S_U8)
+        HVR_UPF(TF_C_F32_F32)
+        HVR_UPF(TF_C_F16_F32)
+        HVR_UPF(TF_C_U8_F32)
+        HVR_UPF(TF_C_F32_SN8)
+        HVR_UPF(TF_C_F16_SN8)
+        HVR_UPF(TF_C_U8_SN8)
+    default: return fail(HVR_EINVAL, "the volume has no scalar field");
+    }
+#undef HVR_UPF
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    v->scalar_loaded = true;
+    if (normals && fmt_has_normal(v->fmt)) v->normals_loaded = true;
+    ++v->gen;
+    return compute_range(v);
+}
+
+int hvr_volume_upload_scalar_normals(hvr_volume *v, const float *data)
+{
+    if (!v || !data) return fail(HVR_EINVAL, "null argument");
+    if (!fmt_has_scalar(v->fmt) || !fmt_has_normal(v->fmt))
+        return fail(HVR_ESTATE, "the volume needs scalar and normal fields");
+    if (v->s0 < 2 || v->s1 < 2 || v->s2 < 2)
+        return fail(HVR_EINVAL, "normals need at least 2 voxels along every axis (np.gradient)");
+    hvr_ctx *c = v->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    const size_t bytes = (size_t)v->s0 * v->s1 * v->s2 * sizeof(float);
+    float *d_field = nullptr;
+    HIPCHK(hipMalloc(&d_field, bytes));
+    int rc = HVR_OK;
+    if (hipMemcpy(d_field, data, bytes, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(HVR_EDEVICE, "copying the scalar field to the device failed");
+    if (rc == HVR_OK) rc = volume_from_field(v, d_field, true);
+    (void)hipFree(d_field);
+    return rc;
+}
+
+int hvr_volume_generate_blobs(hvr_volume *v, int n_blobs, const double *centers, const double *sigmas,
+                              const double *amps, double noise, unsigned long long seed)
+{
+    if (!v || (n_blobs > 0 && (!centers || !sigmas || !amps))) return fail(HVR_EINVAL, "null argument");
+    if (n_blobs < 0 || n_blobs > kMaxBlobs) return fail(HVR_EINVAL, "0 to 32 blobs");
+    if (!fmt_has_scalar(v->fmt)) return fail(HVR_ESTATE, "the volume has no scalar field");
+    const bool normals = fmt_has_normal(v->fmt);
+    if (normals && (v->s0 < 2 || v->s1 < 2 || v->s2 < 2))
+        return fail(HVR_EINVAL, "normals need at least 2 voxels along every axis (np.gradient)");
+    hvr_ctx *c = v->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    BlobSpec spec;
+    memset(&spec, 0, sizeof(spec));
+    spec.n = n_blobs;
+    for (int b = 0; b < n_blobs; ++b) {
+        for (int k = 0; k < 3; ++k) spec.c[3 * b + k] = centers[3 * b + k];
+        spec.s[b] = sigmas[b];
+        spec.a[b] = amps[b];
+    }
+    spec.noise = noise;
+    spec.seed = seed;
+    const uint64_t nvox = (uint64_t)v->s0 * v->s1 * v->s2;
+    float *d_field = nullptr;
+    BlobSpec *d_spec = nullptr;
+    HIPCHK(hipMalloc(&d_field, nvox * sizeof(float)));
+    int rc = HVR_OK;
+    if (hipMalloc(&d_spec, sizeof(spec)) != hipSuccess ||
+        hipMemcpy(d_spec, &spec, sizeof(spec), hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(HVR_EDEVICE, "staging the blob parameters failed");
+    if (rc == HVR_OK) {
+        hipLaunchKernelGGL(k_blob_field, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, c->stream, d_field,
+                           v->s0, v->s1, v->s2, (const BlobSpec *)d_spec, nvox);
+        if (hipGetLastError() != hipSuccess) rc = fail(HVR_EDEVICE, "blob field kernel launch failed");
+    }
+    if (rc == HVR_OK) rc = volume_from_field(v, d_field, normals);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(d_spec);
+    (void)hipFree(d_field);
+    return rc;
+}
+
 int hvr_volume_upload_scalar(hvr_volume *v, const float *data)
 {
     if (!v || !data) return fail(HVR_EINVAL, "null argument");

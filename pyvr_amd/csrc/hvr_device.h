@@ -917,10 +917,10 @@ __device__ __forceinline__ int axis_slots(int c, int s, int lb, uint32_t nb, uin
 }
 
 template <int F>
-__device__ __forceinline__ void store_part(char *p, int part, const float *src, uint64_t l)
+__device__ __forceinline__ void store_part(char *p, int part, const float *val)
 {
     if (part == 1) {
-        const float v = src[l];
+        const float v = val[0];
         if constexpr (F == TF_S_F32) *(float *)p = v;
         else if constexpr (F == TF_S_F16) *(__half *)p = __float2half_rn(v);
         else if constexpr (F == TF_S_U8) *(uint8_t *)p = (uint8_t)enc_u8(v);
@@ -931,7 +931,7 @@ __device__ __forceinline__ void store_part(char *p, int part, const float *src, 
         else if constexpr (F == TF_C_F16_SN8) ((uint32_t *)p)[0] = (uint32_t)__half_as_ushort(__float2half_rn(v));
         else if constexpr (F == TF_C_U8_SN8) ((uint8_t *)p)[3] = (uint8_t)enc_u8(v);
     } else {
-        const float nx = src[3 * l], ny = src[3 * l + 1], nz = src[3 * l + 2];
+        const float nx = val[0], ny = val[1], nz = val[2];
         const uint32_t packed = enc_sn8(nx) | (enc_sn8(ny) << 8) | (enc_sn8(nz) << 16);
         if constexpr (F == TF_N_F32 || F == TF_C_F32_F32 || F == TF_C_F16_F32 || F == TF_C_U8_F32) {
             float *q = (float *)p;
@@ -949,17 +949,13 @@ __device__ __forceinline__ void store_part(char *p, int part, const float *src, 
     }
 }
 
+// Write one voxel's part (val: 1 or 3 floats) to every slot holding its
+// texel; (x, y, z) are GL texel coordinates.
 template <int F>
-__global__ void __launch_bounds__(256)
-k_upload(char *__restrict__ dst, int s0, int s1, int s2, int lb, uint32_t nbx, uint32_t nby, uint32_t nbz,
-         uint32_t brick_bytes, const float *__restrict__ src, int part, int z0, uint64_t nvox)
+__device__ __forceinline__ void scatter_texel(char *__restrict__ dst, int x, int y, int z, int s0, int s1, int s2,
+                                              int lb, uint32_t nbx, uint32_t nby, uint32_t nbz,
+                                              uint32_t brick_bytes, int part, const float *val)
 {
-    const uint64_t l = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    if (l >= nvox) return;
-    const uint64_t plane = (uint64_t)s0 * (uint64_t)s1;
-    const int z = (int)(l / plane) + z0;
-    const uint64_t rem = l % plane;
-    const int y = (int)(rem / (uint64_t)s0), x = (int)(rem % (uint64_t)s0);
     uint32_t bx[4], lx[4], by[4], ly[4], bz[4], lz[4];
     const int nx = axis_slots(x, s0, lb, nbx, bx, lx);
     const int ny = axis_slots(y, s1, lb, nby, by, ly);
@@ -971,8 +967,117 @@ k_upload(char *__restrict__ dst, int s0, int s1, int s2, int lb, uint32_t nbx, u
             for (int i = 0; i < nx; ++i) {
                 const uint32_t brick = (bz[k] * nby + by[j]) * nbx + bx[i];
                 const uint32_t local = (lz[k] * E + ly[j]) * E + lx[i];
-                store_part<F>(dst + (uint64_t)brick * brick_bytes + (uint64_t)local * T, part, src, l);
+                store_part<F>(dst + (uint64_t)brick * brick_bytes + (uint64_t)local * T, part, val);
             }
+}
+
+template <int F>
+__global__ void __launch_bounds__(256)
+k_upload(char *__restrict__ dst, int s0, int s1, int s2, int lb, uint32_t nbx, uint32_t nby, uint32_t nbz,
+         uint32_t brick_bytes, const float *__restrict__ src, int part, int z0, uint64_t nvox)
+{
+    const uint64_t l = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (l >= nvox) return;
+    const uint64_t plane = (uint64_t)s0 * (uint64_t)s1;
+    const int z = (int)(l / plane) + z0;
+    const uint64_t rem = l % plane;
+    const int y = (int)(rem / (uint64_t)s0), x = (int)(rem % (uint64_t)s0);
+    scatter_texel<F>(dst, x, y, z, s0, s1, s2, lb, nbx, nby, nbz, brick_bytes, part,
+                     src + (part == 1 ? l : 3 * l));
+}
+
+// Normals of a device float32 field (numpy C order, shape (s0, s1, s2)),
+// exactly as compute_normal_volume (pyvr/datasets/synthetic.py:109-122)
+// computes them from a float32 array: np.gradient along each numpy axis
+// (central differences halved inside, one-sided at the two faces, unit
+// spacing), the norm sqrt((gx^2 + gy^2) + gz^2) summed left to right as
+// numpy does, plus float32(1e-8), and each component divided by it -- all
+// correctly rounded float32 operations.  Every axis has at least 2 voxels.
+__device__ __forceinline__ float np_gradient(const float *__restrict__ f, uint64_t l, int i, int n, uint64_t st)
+{
+    if (i == 0) return __fsub_rn(f[l + st], f[l]);
+    if (i == n - 1) return __fsub_rn(f[l], f[l - st]);
+    return __fmul_rn(__fsub_rn(f[l + st], f[l - st]), 0.5f);  // /2, exact
+}
+
+template <int F>
+__global__ void __launch_bounds__(256)
+k_normals_from_field(char *__restrict__ dst, int s0, int s1, int s2, int lb, uint32_t nbx, uint32_t nby,
+                     uint32_t nbz, uint32_t brick_bytes, const float *__restrict__ field, uint64_t nvox)
+{
+    const uint64_t l = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (l >= nvox) return;
+    // numpy coordinates of flat index l
+    const uint64_t n12 = (uint64_t)s1 * (uint64_t)s2;
+    const int i0 = (int)(l / n12);
+    const uint64_t r12 = l % n12;
+    const int i1 = (int)(r12 / (uint64_t)s2), i2 = (int)(r12 % (uint64_t)s2);
+    const float gx = np_gradient(field, l, i0, s0, n12);
+    const float gy = np_gradient(field, l, i1, s1, (uint64_t)s2);
+    const float gz = np_gradient(field, l, i2, s2, 1);
+    const float nn = __fadd_rn(__fsqrt_rn(__fadd_rn(__fadd_rn(__fmul_rn(gx, gx), __fmul_rn(gy, gy)), __fmul_rn(gz, gz))),
+                               1e-8f);
+    const float val[3] = {__fdiv_rn(gx, nn), __fdiv_rn(gy, nn), __fdiv_rn(gz, nn)};
+    // GL texel coordinates of the same flat index
+    const uint64_t plane = (uint64_t)s0 * (uint64_t)s1;
+    const int z = (int)(l / plane);
+    const uint64_t rem = l % plane;
+    const int y = (int)(rem / (uint64_t)s0), x = (int)(rem % (uint64_t)s0);
+    scatter_texel<F>(dst, x, y, z, s0, s1, s2, lb, nbx, nby, nbz, brick_bytes, 2, val);
+}
+
+// Seeded Gaussian-blob field generated on the device (numpy C order): the
+// family of pyvr_amd.datasets.blob_field -- axes linspace(-1, 1, s),
+// amplitude * exp(-(axis - centre)^2 / (2 sigma^2)) per axis multiplied as
+// amp * (fx * (fy * fz)) in double, summed over the blobs -- plus uniform
+// noise in [-noise, noise] from a counter-based hash of (seed, voxel),
+// clipped to [0, 1] and rounded to float32.
+static constexpr int kMaxBlobs = 32;
+struct BlobSpec {
+    double c[3 * kMaxBlobs], s[kMaxBlobs], a[kMaxBlobs];
+    double noise;
+    uint64_t seed;
+    int n;
+};
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z)
+{
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ double np_linspace(int i, int n)
+{
+    if (n <= 1) return -1.0;
+    if (i == n - 1) return 1.0;
+    return (double)i * (2.0 / (double)(n - 1)) + -1.0;
+}
+
+__global__ void __launch_bounds__(256)
+k_blob_field(float *__restrict__ out, int s0, int s1, int s2, const BlobSpec *__restrict__ spec, uint64_t nvox)
+{
+    const uint64_t l = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (l >= nvox) return;
+    const uint64_t n12 = (uint64_t)s1 * (uint64_t)s2;
+    const int i0 = (int)(l / n12);
+    const uint64_t r12 = l % n12;
+    const int i1 = (int)(r12 / (uint64_t)s2), i2 = (int)(r12 % (uint64_t)s2);
+    const double x = np_linspace(i0, s0), y = np_linspace(i1, s1), z = np_linspace(i2, s2);
+    double acc = 0.0;
+    for (int b = 0; b < spec->n; ++b) {
+        const double k = 2.0 * spec->s[b] * spec->s[b];
+        const double dx = x - spec->c[3 * b], dy = y - spec->c[3 * b + 1], dz = z - spec->c[3 * b + 2];
+        const double fx = exp(-(dx * dx) / k), fy = exp(-(dy * dy) / k), fz = exp(-(dz * dz) / k);
+        acc += spec->a[b] * (fx * (fy * fz));
+    }
+    if (spec->noise > 0.0) {
+        const uint64_t h = splitmix64(spec->seed ^ splitmix64(l));
+        const double u = (double)(h >> 11) * 0x1p-53;  // [0, 1)
+        acc += (2.0 * u - 1.0) * spec->noise;
+    }
+    out[l] = (float)fmin(fmax(acc, 0.0), 1.0);
 }
 
 // ---------------------------------------------------------------------------

@@ -29,7 +29,7 @@ EXPORTS = (
     "hvr_timing_begin", "hvr_timing_end", "hvr_device_copy", "hvr_set_empty_skip",
     "hvr_skip_info", "hvr_batch_create", "hvr_batch_destroy", "hvr_batch_clear", "hvr_batch_set_tile_shape",
     "hvr_batch_add", "hvr_batch_run_async", "hvr_batch_wait", "hvr_batch_run", "hvr_batch_output",
-    "hvr_batch_counts", "hvr_batch_last_ms",
+    "hvr_batch_counts", "hvr_batch_last_ms", "hvr_volume_upload_scalar_normals", "hvr_volume_generate_blobs",
 )
 
 # batch ray-tile shapes (hvr_batch_set_tile_shape)
@@ -103,6 +103,9 @@ def _declare(L):
         "hvr_batch_output": ([vp, i, ctypes.POINTER(u8p)], i),
         "hvr_batch_counts": ([vp, i, ctypes.POINTER(ctypes.POINTER(ctypes.c_uint32))], i),
         "hvr_batch_last_ms": ([vp, fp], i),
+        "hvr_volume_upload_scalar_normals": ([vp, fp], i),
+        "hvr_volume_generate_blobs": ([vp, i, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ctypes.c_double), ctypes.c_double, ctypes.c_ulonglong], i),
         "hvr_render_async_to": ([vp, vp], i),
         "hvr_device_alloc": ([vp, ctypes.c_uint64, ctypes.POINTER(vp)], i),
         "hvr_device_free": ([vp, vp], i),

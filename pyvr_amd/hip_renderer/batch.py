@@ -16,7 +16,8 @@ renderer's ``render()`` output, sample counts included.
 """
 
 import ctypes
-from typing import Dict, List, Optional
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Tuple
 
 import numpy as np
 
@@ -77,19 +78,60 @@ def request_params(volume: Volume, camera: Camera, config: RenderConfig, light: 
     return p
 
 
-class ResidentVolume:
-    """A ``Volume`` uploaded to the device once and shared by requests."""
+@dataclass
+class DeviceBlobVolume:
+    """A volume generated on the device (``hvr_volume_generate_blobs``):
+    the seeded Gaussian-blob family of ``pyvr_amd.datasets.blob_field``
+    (centres, widths and amplitudes drawn exactly as it draws them) with
+    noise from a counter-based hash, normals computed on the device.  It
+    has no host data; it stands in for a ``Volume`` in ``BatchRenderer``."""
+    shape: Tuple[int, int, int]
+    n_blobs: int = 8
+    seed: int = 1
+    noise: float = 1e-3
+    with_normals: bool = True
+    min_bounds: np.ndarray = field(default_factory=lambda: np.array([-1.0, -1.0, -1.0], np.float32))
+    max_bounds: np.ndarray = field(default_factory=lambda: np.array([1.0, 1.0, 1.0], np.float32))
 
-    def __init__(self, L, ctx, volume: Volume, scalar_format: str, normal_format: str):
+    @property
+    def has_normals(self) -> bool:
+        return self.with_normals
+
+    def blob_params(self):
+        from ..datasets.synthetic import blob_params
+        _, centers, sigmas, amps = blob_params(self.n_blobs, self.seed)
+        return centers, sigmas, amps
+
+
+class ResidentVolume:
+    """A ``Volume`` (or ``DeviceBlobVolume``) on the device, shared by
+    requests.  ``device_normals``: compute the normals of a volume without
+    normals on the device (the bytes compute_normal_volume would give)."""
+
+    def __init__(self, L, ctx, volume, scalar_format: str, normal_format: str, device_normals: bool = False):
+        generated = isinstance(volume, DeviceBlobVolume)
+        self.has_normals = volume.has_normals or (device_normals and not generated)
         sf = _FORMAT_ALIASES.get(scalar_format)
-        nf = _FORMAT_ALIASES.get(normal_format) if volume.has_normals else "none"
+        nf = _FORMAT_ALIASES.get(normal_format) if self.has_normals else "none"
         if sf not in _lib.HVR_SCALAR or sf == "none" or nf not in _lib.HVR_NORMAL:
             raise ValueError(f"unsupported formats scalar={scalar_format!r} normal={normal_format!r}")
         self.volume = volume
-        self.device = _DeviceVolume(L, ctx, volume.data.shape, sf, nf)
-        self.device.upload_scalar(volume.data)
-        if volume.has_normals:
-            self.device.upload_normals(volume.normals)
+        shape = tuple(volume.shape) if generated else volume.data.shape
+        self.device = _DeviceVolume(L, ctx, shape, sf, nf)
+        try:
+            if generated:
+                c, sg, am = volume.blob_params()
+                self.device.generate_blobs(c, sg, am, volume.noise, volume.seed)
+            elif volume.has_normals:
+                self.device.upload_scalar(volume.data)
+                self.device.upload_normals(volume.normals)
+            elif device_normals:
+                self.device.upload_scalar_normals(volume.data)
+            else:
+                self.device.upload_scalar(volume.data)
+        except Exception:
+            self.device.destroy()
+            raise
 
     @property
     def handle(self):
@@ -159,14 +201,24 @@ class BatchRenderer:
         self._last = None    # slot of the last run
         self._reuse = False  # the slot being filled still holds submitted requests
 
+    def set_tile_shape(self, tile_shape: str):
+        """Ray tile of a wavefront for the following launches ("8x8" or "16x4")."""
+        if tile_shape not in TILE_SHAPES:
+            raise ValueError(f"tile_shape must be one of {sorted(TILE_SHAPES)}")
+        for s in self._slots:
+            check(self.L.hvr_batch_set_tile_shape(s.handle, TILE_SHAPES[tile_shape]), "hvr_batch_set_tile_shape")
+        self.tile_shape = tile_shape
+
     # -- volumes -------------------------------------------------------------
-    def upload(self, volume: Volume) -> ResidentVolume:
-        """Make ``volume`` resident (once per Volume object)."""
-        if not isinstance(volume, Volume):
+    def upload(self, volume, device_normals: bool = False) -> ResidentVolume:
+        """Make ``volume`` (a ``Volume`` or ``DeviceBlobVolume``) resident,
+        once per object.  ``device_normals``: a volume without normals gets
+        normals computed on the device."""
+        if not isinstance(volume, (Volume, DeviceBlobVolume)):
             raise TypeError(f"Expected Volume instance, got {type(volume)}")
         rv = self._resident.get(id(volume))
         if rv is None or rv.volume is not volume:
-            rv = ResidentVolume(self.L, self.ctx, volume, self.scalar_format, self.normal_format)
+            rv = ResidentVolume(self.L, self.ctx, volume, self.scalar_format, self.normal_format, device_normals)
             self._resident[id(volume)] = rv
         return rv
 
@@ -208,7 +260,7 @@ class BatchRenderer:
             self._reuse = False
         # normals from the volume's interleaved field; without normals the
         # normal sampler reads the scalar texture (unit 0) as (rho, 0, 0)
-        nrole = _lib.HVR_ROLE_NORMAL if volume.has_normals else _lib.HVR_ROLE_SCALAR
+        nrole = _lib.HVR_ROLE_NORMAL if rv.has_normals else _lib.HVR_ROLE_SCALAR
         slot = self._slots[self._cur]
         idx = ctypes.c_int()
         check(self.L.hvr_batch_add(slot.handle, rv.handle, _lib.HVR_ROLE_SCALAR, rv.handle, nrole,

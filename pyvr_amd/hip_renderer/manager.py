@@ -100,6 +100,28 @@ class _DeviceVolume:
               "hvr_volume_upload_normals")
         self.normals_loaded = True
 
+    def upload_scalar_normals(self, data: np.ndarray):
+        """Scalar field, and its normals computed on the device exactly as
+        compute_normal_volume computes them from the float32 array."""
+        a = np.ascontiguousarray(data, dtype=np.float32)
+        check(self.L.hvr_volume_upload_scalar_normals(self.handle,
+                                                      a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+              "hvr_volume_upload_scalar_normals")
+        self.normals_loaded = True
+
+    def generate_blobs(self, centers, sigmas, amps, noise: float, seed: int):
+        """Seeded Gaussian-blob field (and normals, if held) built on the device."""
+        c = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1, 3)
+        sg = np.ascontiguousarray(sigmas, dtype=np.float64).reshape(-1)
+        am = np.ascontiguousarray(amps, dtype=np.float64).reshape(-1)
+        if not (len(c) == len(sg) == len(am)):
+            raise ValueError("centers, sigmas and amps must describe the same blobs")
+        dp = ctypes.POINTER(ctypes.c_double)
+        check(self.L.hvr_volume_generate_blobs(self.handle, len(sg), c.ctypes.data_as(dp), sg.ctypes.data_as(dp),
+                                               am.ctypes.data_as(dp), float(noise), int(seed)),
+              "hvr_volume_generate_blobs")
+        self.normals_loaded = self.normal_format != "none"
+
     def nbytes(self) -> int:
         v = ctypes.c_uint64()
         check(self.L.hvr_volume_bytes(self.handle, ctypes.byref(v)), "hvr_volume_bytes")
@@ -180,21 +202,28 @@ class HipManager:
         return unit
 
     def create_volume_texture(self, volume_data, scalar_format: str = "f32",
-                              normal_format: Optional[str] = None) -> int:
+                              normal_format: Optional[str] = None, device_normals: bool = False) -> int:
         """Upload a 3-D scalar field (reference manager.py:77-108) and return
         its texture unit.  Any dtype is converted to float32 first.
         ``scalar_format`` selects the device storage ("f32", "f16", "u8");
         ``normal_format`` ("f32x3", "sn8x3") reserves an interleaved normal
-        field that the next ``create_normal_texture`` of the same shape fills."""
+        field that the next ``create_normal_texture`` of the same shape fills,
+        or, with ``device_normals``, that the device computes from the field
+        (``create_device_normal_texture`` then binds it)."""
         if len(volume_data.shape) != 3:
             raise ValueError("Volume data must be 3D")
         sf = _FORMAT_ALIASES.get(scalar_format)
         nf = _FORMAT_ALIASES.get(normal_format)
         if sf not in ("f32", "f16", "u8") or (nf not in (None, "f32x3", "sn8x3")):
             raise ValueError(f"unsupported formats scalar={scalar_format!r} normal={normal_format!r}")
+        if device_normals and nf is None:
+            raise ValueError("device_normals needs a normal_format")
         dv = _DeviceVolume(self.L, self.ctx, volume_data.shape, sf, nf or "none")
         self._volumes.append(dv)
-        dv.upload_scalar(np.asarray(volume_data, dtype=np.float32))
+        if device_normals:
+            dv.upload_scalar_normals(np.asarray(volume_data, dtype=np.float32))
+        else:
+            dv.upload_scalar(np.asarray(volume_data, dtype=np.float32))
         unit = self._add_texture(_Texture3D(dv, _lib.HVR_ROLE_SCALAR, 1))
         self._texture_bindings["volume"] = (self._units[unit], unit)
         return unit
@@ -222,6 +251,17 @@ class HipManager:
             self._volumes.append(target)
         target.upload_normals(data)
         unit = self._add_texture(_Texture3D(target, _lib.HVR_ROLE_NORMAL, 3))
+        self._texture_bindings["normal"] = (self._units[unit], unit)
+        return unit
+
+    def create_device_normal_texture(self) -> int:
+        """Texture unit of the normals the device computed for the newest
+        volume texture (``create_volume_texture(..., device_normals=True)``)."""
+        newest = self._units.get(self._newest_unit)
+        if not (isinstance(newest, _Texture3D) and newest.role == _lib.HVR_ROLE_SCALAR
+                and newest.volume.normals_loaded):
+            raise RuntimeError("the newest volume texture has no device-computed normals")
+        unit = self._add_texture(_Texture3D(newest.volume, _lib.HVR_ROLE_NORMAL, 3))
         self._texture_bindings["normal"] = (self._units[unit], unit)
         return unit
 

@@ -68,12 +68,15 @@ class HipVolumeRenderer:
         self._update_light()
 
     def load_volume(self, volume: Volume, *, scalar_format: Optional[str] = None,
-                    normal_format: Optional[str] = None) -> None:
+                    normal_format: Optional[str] = None, device_normals: bool = False) -> None:
         """Upload ``volume`` (data, normals if present) and its bounds.
 
         Raises ``TypeError`` if ``volume`` is not a ``Volume``.  Without
         normals the normal sampler keeps its previous unit (by default unit
-        0), as in the reference."""
+        0), as in the reference.  ``device_normals=True`` on a volume without
+        normals computes them on the GPU instead -- the same bytes as
+        ``volume.compute_normals()`` followed by ``load_volume``, without the
+        host gradient."""
         if not isinstance(volume, Volume):
             raise TypeError(
                 f"Expected Volume instance, got {type(volume)}. "
@@ -83,14 +86,18 @@ class HipVolumeRenderer:
         self.volume = volume
         sf = scalar_format or self.scalar_format
         nf = normal_format or self.normal_format
+        on_device = device_normals and not volume.has_normals
         unit = self.gl_manager.create_volume_texture(
-            volume.data, scalar_format=sf, normal_format=nf if volume.has_normals else None)
+            volume.data, scalar_format=sf, normal_format=nf if (volume.has_normals or on_device) else None,
+            device_normals=on_device)
         self.gl_manager.set_uniform_int("volume_texture", unit)
         self.gl_manager.set_uniform_vector("volume_min_bounds", tuple(volume.min_bounds))
         self.gl_manager.set_uniform_vector("volume_max_bounds", tuple(volume.max_bounds))
         if volume.has_normals:
             normal_unit = self.gl_manager.create_normal_texture(volume.normals, normal_format=nf)
             self.gl_manager.set_uniform_int("normal_volume", normal_unit)
+        elif on_device:
+            self.gl_manager.set_uniform_int("normal_volume", self.gl_manager.create_device_normal_texture())
 
     def set_camera(self, camera: Camera) -> None:
         """View/projection matrices and camera position from ``camera``
