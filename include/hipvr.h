@@ -94,6 +94,10 @@ const char *hvr_last_error(void);
 /* Number of visible HIP devices. */
 int hvr_device_count(int *count);
 
+/* Free and total device memory in bytes (no counterpart in the reference;
+ * for memory-budgeted serving). */
+int hvr_device_memory(int device, unsigned long long *free_bytes, unsigned long long *total_bytes);
+
 /* ---- context: replaces ModernGLManager.__init__ (manager.py:14-41:
  * standalone context + RGBA8 framebuffer of width x height) ----
  * The march reads volumes below 4 GiB through buffer loads (32-bit
@@ -239,7 +243,8 @@ int hvr_set_brick_log2(hvr_ctx *ctx, int log2_edge);
  * bindings, LUT and parameters (byte for byte, sample counts included).
  *
  * A request reads its scalar field from an uploaded volume of the batch's
- * context with brick edge 8 or 16, below 4 GiB, and its normals from the
+ * context with brick edge 8 or 16 (volumes of 4 GiB and more are read with
+ * 64-bit loads, below with buffer loads), and its normals from the
  * same volume (HVR_ROLE_NORMAL), its scalar (HVR_ROLE_SCALAR: (rho, 0, 0)),
  * or none (n = NULL); other bindings and LUTs of more than 2048 entries
  * need the single-request path.  Volumes must outlive the requests that
@@ -262,6 +267,9 @@ int hvr_batch_add(hvr_batch *batch, hvr_volume *scalar_vol, int scalar_role, hvr
  * hvr_batch_run waits for them. */
 int hvr_batch_run_async(hvr_batch *batch, int count);
 int hvr_batch_wait(hvr_batch *batch);
+/* Non-blocking: *done = 1 when the last run's images (and counts) are in
+ * host memory (also when the batch has not been run), else 0. */
+int hvr_batch_query(hvr_batch *batch, int *done);
 int hvr_batch_run(hvr_batch *batch, int count);
 /* After the run completed: request `index`'s width*height*4 RGBA8 bytes
  * (bottom row first) / its per-pixel sample counts, valid until the next

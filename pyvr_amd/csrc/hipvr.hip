@@ -412,6 +412,16 @@ int hvr_device_count(int *count)
     return HVR_OK;
 }
 
+int hvr_device_memory(int device, unsigned long long *free_bytes, unsigned long long *total_bytes)
+{
+    HIPCHK(hipSetDevice(device));
+    size_t f = 0, t = 0;
+    HIPCHK(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return HVR_OK;
+}
+
 int hvr_create(int device, int width, int height, hvr_ctx **out)
 {
     if (!out) return fail(HVR_EINVAL, "out is null");
@@ -1040,7 +1050,7 @@ struct hvr_batch {
     hipGraph_t graph;
     hipGraphExec_t exec;
     std::string key;                      // shape the graph was captured for
-    hipEvent_t ev0, ev1;
+    hipEvent_t ev0, ev1, ev2;             // graph start / end, copies back done
     bool ran, counted;
     size_t out_pixels;
     int ncu;
@@ -1048,41 +1058,43 @@ struct hvr_batch {
 
 typedef int (*BatchLaunchFn)(const char *blk, int cls, uint32_t *ctr, int lut_cap, int ncu, hipStream_t st);
 
-template <int SF, int NM, int LB, bool COUNT, int TS>
+template <int SF, int NM, int LB, bool COUNT, int TS, bool BUF>
 static int batch_launch(const char *blk, int cls, uint32_t *ctr, int lut_cap, int ncu, hipStream_t st)
 {
     const size_t lds = 2 * (size_t)lut_cap * sizeof(float4);
     int per = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch<SF, NM, LB, COUNT, TS>, 256, lds) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch<SF, NM, LB, COUNT, TS, BUF>, 256, lds) !=
+            hipSuccess ||
         per < 1)
         per = 1;
-    hipLaunchKernelGGL((k_batch<SF, NM, LB, COUNT, TS>), dim3(per * ncu), dim3(256), lds, st, blk, cls, ctr, lut_cap);
+    hipLaunchKernelGGL((k_batch<SF, NM, LB, COUNT, TS, BUF>), dim3(per * ncu), dim3(256), lds, st, blk, cls, ctr,
+                       lut_cap);
     return hipGetLastError() == hipSuccess ? HVR_OK : HVR_EDEVICE;
 }
 
-template <int SF, int NM, int LB, bool COUNT>
+template <int SF, int NM, int LB, bool COUNT, bool BUF>
 static BatchLaunchFn batch_pick_ts(int ts)
 {
-    return ts == HVR_TILE_16X4 ? batch_launch<SF, NM, LB, COUNT, 1> : batch_launch<SF, NM, LB, COUNT, 0>;
+    return ts == HVR_TILE_16X4 ? batch_launch<SF, NM, LB, COUNT, 1, BUF> : batch_launch<SF, NM, LB, COUNT, 0, BUF>;
 }
 
-template <int SF, int NM, bool COUNT>
+template <int SF, int NM, bool COUNT, bool BUF>
 static BatchLaunchFn batch_pick_lb(int lb, int ts)
 {
-    return lb == 4 ? batch_pick_ts<SF, NM, 4, COUNT>(ts) : batch_pick_ts<SF, NM, 3, COUNT>(ts);
+    return lb == 4 ? batch_pick_ts<SF, NM, 4, COUNT, BUF>(ts) : batch_pick_ts<SF, NM, 3, COUNT, BUF>(ts);
 }
 
-template <bool COUNT>
+template <bool COUNT, bool BUF>
 static BatchLaunchFn batch_fn_c(int fmt, int nm, int lb, int ts)
 {
 #define HVR_S(F)                                                                   \
     case F:                                                                        \
-        if (nm == NM_ZERO) return batch_pick_lb<F, NM_ZERO, COUNT>(lb, ts);        \
-        if (nm == NM_ALIAS) return batch_pick_lb<F, NM_ALIAS, COUNT>(lb, ts);      \
+        if (nm == NM_ZERO) return batch_pick_lb<F, NM_ZERO, COUNT, BUF>(lb, ts);   \
+        if (nm == NM_ALIAS) return batch_pick_lb<F, NM_ALIAS, COUNT, BUF>(lb, ts); \
         return nullptr;
 #define HVR_C(F)                                                                   \
     case F:                                                                        \
-        if (nm == NM_SAME) return batch_pick_lb<F, NM_SAME, COUNT>(lb, ts);        \
+        if (nm == NM_SAME) return batch_pick_lb<F, NM_SAME, COUNT, BUF>(lb, ts);   \
         return nullptr;
     switch (fmt) {
         HVR_S(TF_S_F32)
@@ -1100,9 +1112,16 @@ static BatchLaunchFn batch_fn_c(int fmt, int nm, int lb, int ts)
 #undef HVR_C
 }
 
-static BatchLaunchFn batch_fn(int fmt, int nm, int lb, bool count, int ts)
+// buf: the volume is read through buffer loads (below 4 GiB and 2^24 bricks)
+static BatchLaunchFn batch_fn(int fmt, int nm, int lb, bool count, int ts, bool buf)
 {
-    return count ? batch_fn_c<true>(fmt, nm, lb, ts) : batch_fn_c<false>(fmt, nm, lb, ts);
+    if (buf) return count ? batch_fn_c<true, true>(fmt, nm, lb, ts) : batch_fn_c<false, true>(fmt, nm, lb, ts);
+    return count ? batch_fn_c<true, false>(fmt, nm, lb, ts) : batch_fn_c<false, false>(fmt, nm, lb, ts);
+}
+
+static bool volume_buf(const hvr_volume *v)
+{
+    return v->bytes <= 0xffffffffull && (uint64_t)v->nbx * v->nby * v->nbz < (1ull << 24);
 }
 
 static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
@@ -1152,6 +1171,7 @@ int hvr_batch_create(hvr_ctx *c, hvr_batch **out)
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&b->ev0);
     if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev2, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMalloc(&b->d_counters, kBatchMaxClasses * sizeof(uint32_t));
     if (e == hipSuccess) e = hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, c->device);
     if (e != hipSuccess) {
@@ -1178,6 +1198,7 @@ int hvr_batch_destroy(hvr_batch *b)
     if (b->h_counts) (void)hipHostFree(b->h_counts);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->ev2) (void)hipEventDestroy(b->ev2);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return HVR_OK;
@@ -1212,10 +1233,8 @@ int hvr_batch_add(hvr_batch *b, hvr_volume *s, int srole, hvr_volume *n, int nro
     else if (n == s && nrole == HVR_ROLE_SCALAR) nm = NM_ALIAS;
     else if (n == s && nrole == HVR_ROLE_NORMAL && fmt_has_normal(s->fmt)) nm = NM_SAME;
     else return fail(HVR_EINVAL, "a batch request reads its normals from its scalar volume, or none");
-    if (!batch_fn(s->fmt, nm, s->lb, false, HVR_TILE_8X8))
+    if (!batch_fn(s->fmt, nm, s->lb, false, HVR_TILE_8X8, true))
         return fail(HVR_EINVAL, "this volume layout and normal binding need the single-request path");
-    if (s->bytes > 0xffffffffull || (uint64_t)s->nbx * s->nby * s->nbz >= (1ull << 24))
-        return fail(HVR_EINVAL, "batched volumes must be below 4 GiB (and 2^24 bricks)");
     if (lut_n < 0 || lut_n > kLutLdsMax || (lut_n > 0 && !lut_rgba))
         return fail(HVR_EINVAL, "batched LUTs hold 0 to 2048 entries");
     if (width <= 0 || height <= 0 || (long long)width * height > (1ll << 28))
@@ -1248,17 +1267,21 @@ int hvr_batch_run_async(hvr_batch *b, int count)
     // classes: one kernel instance each, in order of first appearance
     struct Cls {
         int fmt, nm, lb;
+        bool buf;
         std::vector<int> members;
     };
     std::vector<Cls> cls;
     int lut_cap = 1;
     for (int i = 0; i < nreq; ++i) {
         const BatchReq &r = b->reqs[i];
+        const bool buf = volume_buf(r.vol) && !c->force_global;
         int k = 0;
-        while (k < (int)cls.size() && !(cls[k].fmt == r.vol->fmt && cls[k].nm == r.nm && cls[k].lb == r.vol->lb)) ++k;
+        while (k < (int)cls.size() && !(cls[k].fmt == r.vol->fmt && cls[k].nm == r.nm && cls[k].lb == r.vol->lb &&
+                                        cls[k].buf == buf))
+            ++k;
         if (k == (int)cls.size()) {
             if (k == kBatchMaxClasses) return fail(HVR_EINVAL, "a batch mixes more than 8 volume layouts");
-            cls.push_back({r.vol->fmt, r.nm, r.vol->lb, {}});
+            cls.push_back({r.vol->fmt, r.nm, r.vol->lb, buf, {}});
         }
         cls[k].members.push_back(i);
         lut_cap = std::max(lut_cap, (int)(r.lut.size() / 4));
@@ -1339,7 +1362,8 @@ int hvr_batch_run_async(hvr_batch *b, int count)
     std::string key = std::to_string(b->stage_cap) + "/" + std::to_string(lut_cap) + "/" +
                       std::to_string(count) + "/" + std::to_string(b->tile_shape);
     for (const Cls &k : cls)
-        key += "/" + std::to_string(k.fmt) + "." + std::to_string(k.nm) + "." + std::to_string(k.lb);
+        key += "/" + std::to_string(k.fmt) + "." + std::to_string(k.nm) + "." + std::to_string(k.lb) + "." +
+               std::to_string(k.buf);
     if (key != b->key || !b->exec) {
         batch_drop_graph(b);
         HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -1353,7 +1377,7 @@ int hvr_batch_run_async(hvr_batch *b, int count)
             if (hipGetLastError() != hipSuccess) rc = HVR_EDEVICE;
         }
         for (size_t k = 0; k < cls.size() && rc == HVR_OK; ++k)
-            rc = batch_fn(cls[k].fmt, cls[k].nm, cls[k].lb, count != 0, b->tile_shape)(
+            rc = batch_fn(cls[k].fmt, cls[k].nm, cls[k].lb, count != 0, b->tile_shape, cls[k].buf)(
                 b->d_stage, (int)k, b->d_counters + k, lut_cap, b->ncu, b->stream);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(b->stream, &g);
@@ -1370,6 +1394,7 @@ int hvr_batch_run_async(hvr_batch *b, int count)
     HIPCHK(hipEventRecord(b->ev1, b->stream));
     HIPCHK(hipMemcpyAsync(b->h_out, b->d_out, pixels * 4, hipMemcpyDeviceToHost, b->stream));
     if (count) HIPCHK(hipMemcpyAsync(b->h_counts, b->d_counts, pixels * 4, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipEventRecord(b->ev2, b->stream));
     b->out_pixels = pixels;
     b->ran = true;
     b->counted = count != 0;
@@ -1381,6 +1406,21 @@ int hvr_batch_wait(hvr_batch *b)
     if (!b) return fail(HVR_EINVAL, "batch is null");
     HIPCHK(hipSetDevice(b->ctx->device));
     HIPCHK(hipStreamSynchronize(b->stream));
+    return HVR_OK;
+}
+
+int hvr_batch_query(hvr_batch *b, int *done)
+{
+    if (!b || !done) return fail(HVR_EINVAL, "null argument");
+    if (!b->ran) {
+        *done = 1;
+        return HVR_OK;
+    }
+    HIPCHK(hipSetDevice(b->ctx->device));
+    const hipError_t e = hipEventQuery(b->ev2);
+    if (e == hipSuccess) *done = 1;
+    else if (e == hipErrorNotReady) *done = 0;
+    else return fail_hip(e, "hvr_batch_query");
     return HVR_OK;
 }
 

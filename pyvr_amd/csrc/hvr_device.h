@@ -1135,7 +1135,7 @@ struct TileShape<1> {
     static constexpr int WX = 16, WY = 4;
 };
 
-template <int SF, int NM, int LB, bool COUNT, int TS>
+template <int SF, int NM, int LB, bool COUNT, int TS, bool BUF>
 __global__ void __launch_bounds__(256)
 k_batch(const char *__restrict__ block, int cls, uint32_t *__restrict__ counter, int lut_cap)
 {
@@ -1184,8 +1184,8 @@ k_batch(const char *__restrict__ block, int cls, uint32_t *__restrict__ counter,
             const RayRec r = ray_setup(R.fc, px, py);
             float C0 = 0.f, C1 = 0.f, C2 = 0.f, A = 0.f;
             uint32_t ns = 0;
-            march_ray<SF, NM, LB, COUNT, true, true>(R.fc, R.ts, R.ts, R.lut, R.lut_n, s_val, s_dlt, r, R.flags,
-                                                     C0, C1, C2, A, ns);
+            march_ray<SF, NM, LB, COUNT, true, BUF>(R.fc, R.ts, R.ts, R.lut, R.lut_n, s_val, s_dlt, r, R.flags,
+                                                    C0, C1, C2, A, ns);
             const int pix = py * R.fc.W + px;
             st_global(R.out + pix, blend_store(R.fc, C0, C1, C2, A));
             if constexpr (COUNT) st_global(R.counts + pix, ns);

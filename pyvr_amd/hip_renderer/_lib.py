@@ -30,6 +30,7 @@ EXPORTS = (
     "hvr_skip_info", "hvr_batch_create", "hvr_batch_destroy", "hvr_batch_clear", "hvr_batch_set_tile_shape",
     "hvr_batch_add", "hvr_batch_run_async", "hvr_batch_wait", "hvr_batch_run", "hvr_batch_output",
     "hvr_batch_counts", "hvr_batch_last_ms", "hvr_volume_upload_scalar_normals", "hvr_volume_generate_blobs",
+    "hvr_batch_query", "hvr_device_memory",
 )
 
 # batch ray-tile shapes (hvr_batch_set_tile_shape)
@@ -104,6 +105,8 @@ def _declare(L):
         "hvr_batch_counts": ([vp, i, ctypes.POINTER(ctypes.POINTER(ctypes.c_uint32))], i),
         "hvr_batch_last_ms": ([vp, fp], i),
         "hvr_volume_upload_scalar_normals": ([vp, fp], i),
+        "hvr_batch_query": ([vp, ctypes.POINTER(i)], i),
+        "hvr_device_memory": ([i, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_ulonglong)], i),
         "hvr_volume_generate_blobs": ([vp, i, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(ctypes.c_double), ctypes.c_double, ctypes.c_ulonglong], i),
         "hvr_render_async_to": ([vp, vp], i),
@@ -139,6 +142,13 @@ def lib():
         raise RuntimeError(f"cannot load {LIB_PATH}: {e}") from e
     _lib = L
     return _lib
+
+
+def device_memory(device: int = 0):
+    """(free, total) device memory in bytes."""
+    f, t = ctypes.c_ulonglong(), ctypes.c_ulonglong()
+    check(lib().hvr_device_memory(int(device), ctypes.byref(f), ctypes.byref(t)), "hvr_device_memory")
+    return int(f.value), int(t.value)
 
 
 def check(rc: int, what: str = "") -> None:

@@ -313,6 +313,25 @@ class BatchRenderer:
         check(self.L.hvr_batch_wait(s.handle), "hvr_batch_wait")
         s.in_flight = False
 
+    def done(self, slot: int) -> bool:
+        """Non-blocking: whether slot ``slot``'s images are in host memory."""
+        s = self._slots[slot]
+        if not s.in_flight:
+            return True
+        d = ctypes.c_int()
+        check(self.L.hvr_batch_query(s.handle, ctypes.byref(d)), "hvr_batch_query")
+        if d.value:
+            s.in_flight = False
+        return bool(d.value)
+
+    def free_slot(self) -> Optional[int]:
+        """The slot being filled if it is free to fill (its previous batch,
+        if any, finished), else None; never blocks."""
+        k = self._cur
+        if self._reuse and not self.done(k):
+            return None
+        return k
+
     def run(self, count: bool = False):
         """Render every request of the batch being filled (and, with
         ``count``, per-pixel sample counts); returns when the images are in
@@ -326,6 +345,18 @@ class BatchRenderer:
         if k is None:
             raise RuntimeError("no batch has been run")
         return k, self._slots[k]
+
+    def view(self, index: int, slot: int) -> np.ndarray:
+        """Request ``index``'s RGBA8 pixels in slot ``slot`` as a read-only
+        array over pinned host memory (valid until the slot is reused);
+        the slot must have completed."""
+        s = self._slots[slot]
+        w, h = s.sizes[index]
+        ptr = ctypes.POINTER(ctypes.c_uint8)()
+        check(self.L.hvr_batch_output(s.handle, index, ctypes.byref(ptr)), "hvr_batch_output")
+        a = np.ctypeslib.as_array(ptr, shape=(h * w * 4,))
+        a.flags.writeable = False
+        return a
 
     def result(self, index: int, slot: Optional[int] = None) -> bytes:
         """Request ``index``'s RGBA8 bytes, bottom row first, from slot

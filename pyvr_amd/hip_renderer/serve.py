@@ -1,0 +1,245 @@
+"""Serving loop: continuous batching of render requests over volumes kept
+resident in HBM, admitted by a device-memory budget.
+
+The reference has no serving layer (one ``VolumeRenderer.render()`` per
+frame, ``pyvr/moderngl_renderer/renderer.py:209-219``).  ``RenderServer``
+turns a stream of requests into batches for ``BatchRenderer``:
+
+* requests queue in arrival order; whenever a batch slot is free, the
+  queued requests that fit (at most ``max_batch_requests`` and
+  ``max_batch_pixels``; a request whose volume cannot be made resident yet
+  waits while later ones go ahead) are launched together, so requests that
+  arrive while others are on the GPU join the next batch, and up to
+  ``pipeline`` batches run at once;
+* a volume is uploaded (or generated) on first use and stays resident;
+  when the budget would be exceeded, resident volumes that no queued-for-
+  launch or in-flight batch uses are evicted, least recently used first;
+  the budget also reserves the batches' output pools;
+* a request's latency runs from its arrival to its pixels being in host
+  memory; ``poll()`` returns the requests that completed, with ``rgba``
+  filled (the same bytes ``VolumeRenderer.render()`` gives for it) unless
+  ``keep_results`` is off (load measurements that do not consume pixels).
+"""
+
+import time
+from collections import OrderedDict, deque
+from dataclasses import dataclass, field
+from typing import Deque, Dict, Hashable, List, Optional
+
+import numpy as np
+
+from . import _lib
+from .batch import BatchRenderer, DeviceBlobVolume
+
+_TEXEL_BYTES = {("f32", "none"): 4, ("f16", "none"): 2, ("u8", "none"): 1,
+                ("f32", "f32x3"): 16, ("f16", "f32x3"): 16, ("u8", "f32x3"): 16,
+                ("f32", "sn8x3"): 8, ("f16", "sn8x3"): 8, ("u8", "sn8x3"): 4}
+
+
+def volume_device_bytes(shape, scalar_format: str, normal_format: str, has_normals: bool,
+                        brick_log2: int = 3) -> int:
+    """Device bytes of a volume (apron bricks plus the per-brick range), as
+    ``hvr_volume_create`` allocates them."""
+    nf = normal_format if has_normals else "none"
+    t = _TEXEL_BYTES[scalar_format, nf]
+    e = (1 << brick_log2) + 1
+    nb = 1
+    for s in shape:
+        nb *= (int(s) >> brick_log2) + 1
+    return nb * e ** 3 * t + nb * 8
+
+
+@dataclass
+class ServeRequest:
+    """One render request; the server fills the timing fields and ``rgba``."""
+    volume: Hashable                  # key given to RenderServer.register
+    camera: object
+    ctf: object
+    otf: object
+    width: int = 512
+    height: int = 512
+    config: object = None
+    light: object = None
+    tau: float = 0.99
+    arrival: Optional[float] = None   # perf_counter() time; stamped by submit if None
+    launched: Optional[float] = None
+    done: Optional[float] = None
+    rgba: Optional[np.ndarray] = None  # width*height*4 uint8, bottom row first
+    tag: object = None
+
+    @property
+    def latency(self) -> Optional[float]:
+        return None if self.done is None else self.done - self.arrival
+
+
+@dataclass
+class _Resident:
+    volume: object
+    bytes: int
+    pins: int = 0
+
+
+class RenderServer:
+    """Continuous-batching render server on one GPU (see the module doc)."""
+
+    def __init__(self, device: int = 0, scalar_format: str = "f16", normal_format: str = "sn8x3",
+                 brick_log2: int = 3, tile_shape: str = "8x8", memory_budget: Optional[int] = None,
+                 max_batch_requests: int = 256, max_batch_pixels: int = 1 << 25, pipeline: int = 3,
+                 keep_results: bool = True, backend=None):
+        self.scalar_format, self.normal_format, self.brick_log2 = scalar_format, normal_format, brick_log2
+        self.backend = backend if backend is not None else BatchRenderer(
+            device=device, scalar_format=scalar_format, normal_format=normal_format, brick_log2=brick_log2,
+            tile_shape=tile_shape, pipeline=pipeline)
+        if memory_budget is None:
+            free, total = _lib.device_memory(device)
+            memory_budget = int(0.9 * free)
+        self.keep_results = keep_results
+        self.max_batch_requests = max_batch_requests
+        self.max_batch_pixels = max_batch_pixels
+        # output pools of the batches in flight: device RGBA8 per pixel
+        self.reserved = pipeline * max_batch_pixels * 4
+        self.memory_budget = int(memory_budget)
+        if self.memory_budget <= self.reserved:
+            raise ValueError("memory budget does not cover the batches' output pools")
+        self._registry: Dict[Hashable, tuple] = {}
+        self._resident: "OrderedDict[Hashable, _Resident]" = OrderedDict()
+        self._queue: Deque[ServeRequest] = deque()
+        self._flight: Dict[int, List[ServeRequest]] = {}
+        self.batches = 0
+        self.evictions = 0
+
+    # -- volumes -------------------------------------------------------------
+    def register(self, key: Hashable, volume, device_normals: bool = False):
+        """Make ``volume`` (``Volume`` or ``DeviceBlobVolume``) servable under
+        ``key``; it is loaded on first use.  Raises ``ValueError`` if it can
+        never fit the budget."""
+        has_n = volume.has_normals or (device_normals and not isinstance(volume, DeviceBlobVolume))
+        shape = volume.shape if isinstance(volume, DeviceBlobVolume) else volume.data.shape
+        nbytes = volume_device_bytes(shape, self.scalar_format, self.normal_format, has_n, self.brick_log2)
+        if nbytes > self.memory_budget - self.reserved:
+            raise ValueError(f"volume {key!r} needs {nbytes} bytes, more than the budget allows")
+        self._registry[key] = (volume, device_normals, nbytes)
+
+    def preload(self, keys=None):
+        """Load registered volumes now (as many as the budget holds)."""
+        for key in (self._registry if keys is None else keys):
+            if not self._make_resident(key):
+                break
+
+    def resident_bytes(self) -> int:
+        return sum(r.bytes for r in self._resident.values())
+
+    def resident_keys(self) -> List[Hashable]:
+        return list(self._resident)
+
+    def _make_resident(self, key) -> bool:
+        r = self._resident.get(key)
+        if r is not None:
+            self._resident.move_to_end(key)
+            return True
+        volume, device_normals, nbytes = self._registry[key]
+        room = self.memory_budget - self.reserved - self.resident_bytes()
+        if nbytes > room:
+            for k in list(self._resident):            # least recently used first
+                if room >= nbytes:
+                    break
+                v = self._resident[k]
+                if v.pins == 0:
+                    self.backend.release(v.volume)
+                    del self._resident[k]
+                    room += v.bytes
+                    self.evictions += 1
+            if nbytes > room:
+                return False
+        self.backend.upload(volume, device_normals=device_normals)
+        self._resident[key] = _Resident(volume, nbytes)
+        return True
+
+    # -- requests ------------------------------------------------------------
+    def submit(self, req: ServeRequest) -> ServeRequest:
+        if req.volume not in self._registry:
+            raise KeyError(f"unknown volume {req.volume!r}")
+        if req.arrival is None:
+            req.arrival = time.perf_counter()
+        self._queue.append(req)
+        return req
+
+    def pending(self) -> int:
+        return len(self._queue) + sum(len(v) for v in self._flight.values())
+
+    def poll(self) -> List[ServeRequest]:
+        """Collect finished batches, then launch queued requests into free
+        slots; never waits for the GPU.  Returns the completed requests."""
+        out = []
+        for k in list(self._flight):
+            if self.backend.done(k):
+                out.extend(self._collect(k))
+        while self._queue:
+            k = self.backend.free_slot()
+            if k is None:
+                break
+            if k in self._flight:
+                # it finished after the pass above: collect its results
+                # before the slot is refilled
+                out.extend(self._collect(k))
+            if not self._launch():
+                break
+        return out
+
+    def _collect(self, k) -> List[ServeRequest]:
+        now = time.perf_counter()
+        reqs = self._flight.pop(k)
+        for i, r in enumerate(reqs):
+            r.done = now
+            if self.keep_results:
+                r.rgba = np.array(self.backend.view(i, k))
+            self._resident[r.volume].pins -= 1
+        return reqs
+
+    def _launch(self) -> bool:
+        chosen, pixels, waiting = [], 0, deque()
+        while self._queue and len(chosen) < self.max_batch_requests:
+            r = self._queue[0]
+            px = r.width * r.height
+            if chosen and pixels + px > self.max_batch_pixels:
+                break
+            self._queue.popleft()
+            # pin before loading others, so a request's volume is not evicted
+            # by the next request of the same batch
+            if self._make_resident(r.volume):
+                self._resident[r.volume].pins += 1
+                chosen.append(r)
+                pixels += px
+            else:
+                waiting.append(r)
+        self._queue.extendleft(reversed(waiting))
+        if not chosen:
+            return False
+        for r in chosen:
+            vol = self._registry[r.volume][0]
+            self.backend.add(vol, r.camera, r.ctf, r.otf, r.width, r.height, r.config, r.light, r.tau)
+        k = self.backend.submit()
+        now = time.perf_counter()
+        for r in chosen:
+            r.launched = now
+        self._flight[k] = chosen
+        self.batches += 1
+        return True
+
+    def drain(self, sleep: float = 0.0) -> List[ServeRequest]:
+        """Poll until every submitted request completed."""
+        out = []
+        while self.pending():
+            got = self.poll()
+            out.extend(got)
+            if not self._flight and self._queue:
+                # nothing running and nothing launchable: cannot progress
+                raise RuntimeError(f"{len(self._queue)} queued requests cannot be admitted")
+            if not got and sleep:
+                time.sleep(sleep)
+        return out
+
+    def close(self):
+        if self.backend is not None and hasattr(self.backend, "close"):
+            self.backend.close()
+        self.backend = None

@@ -211,6 +211,55 @@ def c3_requests(n: int = 64, seed: int = 3, image_scale: int = 1, nvolumes: int 
 
 
 # ---------------------------------------------------------------------------
+# c4: serving
+
+C4_PRESETS = ("fast", "balanced", "high_quality")
+
+
+def c4_sizes(n: int = 16):
+    """N_k = 256 * 4^(k/15) rounded to a multiple of 16 (256 ... 1024)."""
+    return [int(round(256 * 4 ** (k / (n - 1)) / 16)) * 16 for k in range(n)]
+
+
+def c4_volumes(n: int = 16, seed: int = 4, scale: int = 1):
+    """Device-generated blob volumes (8 blobs, seed 100*seed + k, noise
+    1e-3) of sizes c4_sizes(n) / scale, normals computed on the device."""
+    from pyvr_amd.hip_renderer.batch import DeviceBlobVolume
+
+    return [DeviceBlobVolume(shape=(max(8, s // scale),) * 3, n_blobs=8, seed=100 * seed + k, noise=1e-3)
+            for k, s in enumerate(c4_sizes(n))]
+
+
+def c4_requests(count: int, nvol: int = 16, seed: int = 4, image_scale: int = 1):
+    """Requests drawn from default_rng(400 + seed): volume by a truncated
+    Zipf(1.1) over ranks 1..nvol (rank r -> a seeded permutation of the
+    volumes), square image side U[512, 2048] rounded to 16, preset uniform
+    over fast / balanced / high_quality, camera, OTF and colormap as c3;
+    default termination alpha."""
+    rng = np.random.default_rng(400 + seed)
+    perm = rng.permutation(nvol)
+    p = 1.0 / np.arange(1, nvol + 1) ** 1.1
+    p /= p.sum()
+    out = []
+    for _ in range(count):
+        v = int(perm[rng.choice(nvol, p=p)])
+        side = int(round(rng.uniform(512, 2048) / 16)) * 16
+        preset = C4_PRESETS[int(rng.integers(3))]
+        az = float(rng.uniform(0.0, 2 * np.pi))
+        el = float(rng.uniform(-np.pi / 3, np.pi / 3))
+        d = float(rng.uniform(2.5, 4.0))
+        npts = int(rng.integers(4, 9))
+        otf_seed = int(rng.integers(1 << 20))
+        cmap = C3_COLORMAPS[int(rng.integers(len(C3_COLORMAPS)))]
+        side = max(16, side // image_scale)
+        out.append(Request(volume=v, camera=standard_camera(az, el, d),
+                           ctf=ColorTransferFunction.from_colormap(cmap), otf=seeded_otf(npts, otf_seed),
+                           width=side, height=side, config=getattr(RenderConfig, preset)(),
+                           light=standard_light(), tau=0.99))
+    return out
+
+
+# ---------------------------------------------------------------------------
 # storage-format round trips as the device performs them (float32 results);
 # the oracle consumes these to check a quantised render against the same data
 def dequant_scalar(data: np.ndarray, fmt: str) -> np.ndarray:

@@ -119,6 +119,22 @@ def test_batch_formats_bricks_and_no_normals(hip, sf, nf, lb, normals):
         b.close()
 
 
+def test_batch_global_load_instances(hip, monkeypatch):
+    """The 64-bit-load batch kernels (volumes of 4 GiB and more; forced here
+    with HVR_GATHER=global) give the same bytes and counts as the
+    buffer-load single-request path."""
+    vols = scenes.c3_volumes(scale=4)
+    reqs = scenes.c3_requests(6, seed=13, image_scale=4)
+    monkeypatch.setenv("HVR_GATHER", "global")
+    b, idx = make_batch(vols, reqs)
+    monkeypatch.delenv("HVR_GATHER")
+    try:
+        b.run(count=True)
+        check_against_single(b, idx, vols, reqs, "f16", "sn8x3")
+    finally:
+        b.close()
+
+
 def test_batch_pipeline_slots(hip):
     """Two batches in flight on their own streams: each slot's results equal
     a synchronous run of the same requests."""

@@ -1,0 +1,47 @@
+"""The serving loop on the GPU: requests submitted in waves while earlier
+batches are in flight, under a memory budget that forces evictions; every
+result equals what a fresh single-request renderer gives for it (each
+result is the same alone and in the mix)."""
+
+import time
+
+import numpy as np
+import pytest
+
+import scenes
+from test_gpu_batch import single
+
+pytestmark = pytest.mark.gpu
+
+
+def test_served_results_equal_single_renders(hip):
+    from pyvr_amd.hip_renderer.serve import RenderServer, ServeRequest, volume_device_bytes
+
+    vols = scenes.c3_volumes(scale=4)
+    sizes = sorted(volume_device_bytes(v.data.shape, "f16", "sn8x3", True) for v in vols)
+    pipeline, max_px = 2, 1 << 16
+    budget = pipeline * max_px * 4 + sizes[-1] + sizes[-2] + 1      # two largest volumes at most
+    srv = RenderServer(scalar_format="f16", normal_format="sn8x3", memory_budget=budget,
+                       max_batch_requests=5, max_batch_pixels=max_px, pipeline=pipeline)
+    try:
+        for k, v in enumerate(vols):
+            srv.register(k, v)
+        reqs = scenes.c3_requests(24, seed=41, image_scale=4)
+        sub = []
+        for w in range(3):
+            for q in reqs[8 * w: 8 * w + 8]:
+                sub.append((q, srv.submit(ServeRequest(volume=q.volume, camera=q.camera, ctf=q.ctf, otf=q.otf,
+                                                       width=q.width, height=q.height, config=q.config,
+                                                       light=q.light, tau=q.tau))))
+            srv.poll()
+        t0 = time.perf_counter()
+        srv.drain()
+        print(f"[serve] {srv.batches} batches, {srv.evictions} evictions, drained in "
+              f"{(time.perf_counter() - t0) * 1e3:.1f} ms", flush=True)
+        assert srv.evictions > 0 and srv.batches >= 5
+        for q, r in sub:
+            img, _ = single(q, vols[q.volume], "f16", "sn8x3")
+            assert r.rgba is not None and r.rgba.tobytes() == img
+            assert r.latency >= 0
+    finally:
+        srv.close()

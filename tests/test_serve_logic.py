@@ -1,0 +1,198 @@
+"""RenderServer scheduling on the CPU (pyvr_amd/hip_renderer/serve.py) with a
+stand-in backend: FIFO batches within the request and pixel limits,
+continuous batching into free slots, least-recently-used eviction that
+never touches volumes in use, requests waiting for memory while later ones
+proceed, and the footprint rule of hvr_volume_create."""
+
+from types import SimpleNamespace
+
+import numpy as np
+import pytest
+
+from pyvr_amd.hip_renderer.serve import RenderServer, ServeRequest, volume_device_bytes
+
+
+class FakeBackend:
+    def __init__(self, pipeline=2):
+        self.slots = [[] for _ in range(pipeline)]
+        self.busy = [False] * pipeline
+        self.cur = 0
+        self.reuse = False
+        self.resident = set()
+        self.uploads, self.releases, self.launches = [], [], []
+
+    def upload(self, v, device_normals=False):
+        assert id(v) not in self.resident
+        self.resident.add(id(v))
+        self.uploads.append(v.name)
+
+    def release(self, v):
+        self.resident.remove(id(v))
+        self.releases.append(v.name)
+
+    def free_slot(self):
+        # as BatchRenderer.free_slot: asks done() (which may see it finish)
+        return None if (self.reuse and not self.done(self.cur)) else self.cur
+
+    def add(self, vol, camera, ctf, otf, w, h, config, light, tau):
+        assert id(vol) in self.resident, "request on a volume that is not resident"
+        if self.reuse:
+            assert not self.busy[self.cur]
+            self.slots[self.cur] = []
+            self.reuse = False
+        self.slots[self.cur].append((vol.name, w, h))
+
+    def submit(self):
+        k = self.cur
+        self.busy[k] = True
+        self.launches.append(list(self.slots[k]))
+        self.cur = (k + 1) % len(self.slots)
+        self.reuse = True
+        return k
+
+    def done(self, k):
+        return not self.busy[k]
+
+    def finish(self, k):
+        self.busy[k] = False
+
+    def view(self, i, k):
+        _, w, h = self.slots[k][i]
+        return np.zeros(w * h * 4, np.uint8)
+
+
+def vol(name, n):
+    return SimpleNamespace(name=name, data=np.zeros((n, n, n), np.float32), has_normals=True)
+
+
+def req(key, w=16, h=16):
+    return ServeRequest(volume=key, camera=None, ctf=None, otf=None, width=w, height=h)
+
+
+B16 = volume_device_bytes((16, 16, 16), "f16", "sn8x3", True)     # 3^3 bricks of 729 texels * 8 B + range
+
+
+def test_footprint_rule():
+    assert B16 == 27 * 729 * 8 + 27 * 8
+    assert volume_device_bytes((96, 160, 224), "u8", "sn8x3", False, 4) == 7 * 11 * 15 * 17 ** 3 * 1 + 7 * 11 * 15 * 8
+
+
+def make(budget_volumes, pipeline=2, **kw):
+    be = FakeBackend(pipeline)
+    reserve = pipeline * kw.get("max_batch_pixels", 1 << 12) * 4
+    kw.setdefault("max_batch_pixels", 1 << 12)
+    srv = RenderServer(backend=be, memory_budget=reserve + budget_volumes * B16 + 1, pipeline=pipeline, **kw)
+    for k in "ABC":
+        srv.register(k, vol(k, 16))
+    return srv, be
+
+
+def test_fifo_batches_and_continuous_batching():
+    srv, be = make(3, max_batch_requests=3)
+    for k in "ABCAB":
+        srv.submit(req(k))
+    assert srv.poll() == []
+    # two slots: the first takes 3 requests, the second the remaining 2
+    assert [[n for n, _, _ in l] for l in be.launches] == [["A", "B", "C"], ["A", "B"]]
+    srv.submit(req("C"))
+    assert srv.poll() == [] and len(be.launches) == 2        # no free slot: C waits
+    be.finish(0)
+    done = srv.poll()
+    assert [r.volume for r in done] == ["A", "B", "C"] and all(r.rgba.size == 16 * 16 * 4 for r in done)
+    assert [n for n, _, _ in be.launches[-1]] == ["C"]       # joined the freed slot
+    be.finish(1)
+    be.finish(0)
+    assert len(srv.drain()) == 3 and srv.pending() == 0
+    assert all(r.latency is not None and r.latency >= 0 for r in done)
+
+
+def test_pixel_limit_splits_batches():
+    srv, be = make(3, max_batch_pixels=1000)
+    for k in "AB":
+        srv.submit(req(k, 20, 20))                           # 400 px each
+    srv.submit(req("C", 20, 20))
+    srv.poll()
+    assert [len(l) for l in be.launches] == [2, 1]
+
+
+def test_lru_eviction_never_touches_volumes_in_use():
+    srv, be = make(2, max_batch_requests=1)
+    srv.submit(req("A"))
+    srv.submit(req("B"))
+    srv.poll()                                               # A and B in flight, both pinned
+    srv.submit(req("C"))
+    srv.poll()
+    assert be.releases == [] and len(be.launches) == 2       # C waits: nothing evictable, no slot
+    be.finish(0)
+    srv.poll()                                               # A done and unpinned; C evicts A (LRU)
+    assert be.releases == ["A"] and be.launches[-1][0][0] == "C"
+    be.finish(1)
+    be.finish(0)
+    srv.drain()
+    srv.submit(req("B"))                                     # resident: no upload
+    srv.poll()
+    assert be.uploads == ["A", "B", "C"]
+
+
+def test_waiting_request_lets_later_ones_pass():
+    srv, be = make(1, max_batch_requests=4, pipeline=2)
+    srv.submit(req("A"))
+    srv.poll()                                               # A resident and pinned
+    srv.submit(req("B"))                                     # B cannot load while A is pinned
+    srv.submit(req("A"))
+    srv.poll()
+    assert [n for n, _, _ in be.launches[1]] == ["A"]        # the later A request went ahead
+    assert [r.volume for r in srv._queue] == ["B"]
+    be.finish(0)
+    be.finish(1)
+    assert [r.volume for r in srv.poll()] == ["A", "A"]      # both done; B evicts A and launches
+    assert be.releases == ["A"] and be.uploads == ["A", "B"]
+    be.finish(0)
+    assert [r.volume for r in srv.poll()] == ["B"] and srv.pending() == 0
+
+
+class LateBackend(FakeBackend):
+    """A slot finishes on its own after a number of completion queries, so a
+    batch can finish between poll()'s collection pass and its launch pass."""
+
+    def __init__(self, pipeline=2, after=2):
+        super().__init__(pipeline)
+        self.after = after
+        self.queries = [0] * pipeline
+
+    def submit(self):
+        self.queries[self.cur] = 0
+        return super().submit()
+
+    def done(self, k):
+        if self.busy[k]:
+            self.queries[k] += 1
+            if self.queries[k] > self.after:
+                self.busy[k] = False
+        return not self.busy[k]
+
+
+@pytest.mark.parametrize("after", [0, 1, 2, 3])
+def test_no_request_lost_when_a_batch_finishes_mid_poll(after):
+    be = LateBackend(2, after)
+    srv = RenderServer(backend=be, memory_budget=2 * (1 << 12) * 4 + 3 * B16 + 1, pipeline=2,
+                       max_batch_requests=2, max_batch_pixels=1 << 12)
+    for k in "ABC":
+        srv.register(k, vol(k, 16))
+    reqs = [srv.submit(req(k)) for k in "ABCABCABCA"]
+    done = []
+    for _ in range(1000):
+        done.extend(srv.poll())
+        if not srv.pending():
+            break
+    assert sorted(id(r) for r in done) == sorted(id(r) for r in reqs)
+    assert all(r.done is not None for r in reqs)
+    assert all(v.pins == 0 for v in srv._resident.values())
+
+
+def test_rejects_what_can_never_fit():
+    srv, _ = make(1)
+    with pytest.raises(ValueError):
+        srv.register("big", vol("big", 64))
+    with pytest.raises(KeyError):
+        srv.submit(req("nope"))
