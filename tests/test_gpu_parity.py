@@ -160,3 +160,22 @@ def test_transparent_brick_skip_with_nonfinite_lut(hip):
             assert np.array_equal(a, b) and np.array_equal(ca, cb), (entry, value)
     finally:
         r.close()
+
+
+@pytest.mark.parametrize("name", ["hydrogen", "fuel"])
+def test_vti_example_files(hip, name):
+    """Volumes read from the reference's example .vti files
+    (load_vtk_volume) render to the oracle's bar."""
+    from pathlib import Path
+
+    from pyvr_amd.dataloaders import load_vtk_volume
+
+    v = load_vtk_volume(Path(__file__).parent / "golden" / "example_data" / f"{name}.vti")
+    base = scenes.c0(size=8, image=128)
+    sc = scenes.Scene(name=f"vti_{name}", data=v.data, normals=v.normals, camera=base.camera,
+                      light=base.light, ctf=base.ctf, otf=scenes.seeded_otf(6, seed=5), config=base.config,
+                      width=128, height=128, tau=0.99, bmin=v.min_bounds, bmax=v.max_bounds)
+    img = render_hip(sc)
+    ref = oracle_render(sc)
+    assert_parity(img, ref["rgba8"], label=sc.name)
+    assert ref["rgba8"][..., 3].max() > 0
