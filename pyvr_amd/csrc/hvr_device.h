@@ -347,6 +347,68 @@ __device__ __forceinline__ float4 decode_words(const uint32_t *w)
     }
 }
 
+// Scalar (raw, as decode_words' w) and normal of a combined-format texel.
+template <int F>
+__device__ __forceinline__ float decode_scalar(const uint32_t *w)
+{
+    if constexpr (F == TF_C_F32_F32) return __uint_as_float(w[3]);
+    else if constexpr (F == TF_C_F16_F32) return __half2float(__ushort_as_half((unsigned short)(w[3] & 0xffffu)));
+    else if constexpr (F == TF_C_U8_F32) return (float)(w[3] & 0xffu);
+    else if constexpr (F == TF_C_F32_SN8) return __uint_as_float(w[0]);
+    else if constexpr (F == TF_C_F16_SN8) return __half2float(__ushort_as_half((unsigned short)(w[0] & 0xffffu)));
+    else if constexpr (F == TF_C_U8_SN8) return (float)(w[0] >> 24);
+    else return 0.f;
+}
+template <int F>
+__device__ __forceinline__ float3 decode_normal(const uint32_t *w)
+{
+    if constexpr (F == TF_C_F32_F32 || F == TF_C_F16_F32 || F == TF_C_U8_F32)
+        return make_float3(__uint_as_float(w[0]), __uint_as_float(w[1]), __uint_as_float(w[2]));
+    else if constexpr (F == TF_C_F32_SN8 || F == TF_C_F16_SN8)
+        return make_float3(sn8f(w[1], 0), sn8f(w[1], 8), sn8f(w[1], 16));
+    else if constexpr (F == TF_C_U8_SN8)
+        return make_float3(sn8f(w[0], 0), sn8f(w[0], 8), sn8f(w[0], 16));
+    else return make_float3(0.f, 0.f, 0.f);
+}
+
+// The raw dwords of a footprint's eight texels of a combined format (4
+// bytes and more): pair k = taps 2k, 2k+1 (y = k & 1, z = k >> 1).
+template <int F, int LB, bool BUF>
+__device__ __forceinline__ void load_pairs(const Src<BUF> &s, const Footprint &f, uint32_t raw[4][texel_bytes(F) / 2])
+{
+    constexpr int T = texel_bytes(F);
+    constexpr uint32_t E = (1u << LB) + 1u;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        load_words<T / 2, BUF>(s, f, ((uint32_t)(k & 1) * E + (uint32_t)(k >> 1) * E * E) * (uint32_t)T, raw[k]);
+}
+
+// Trilinear scalar of a combined-format footprint from its raw pairs.
+template <int F>
+__device__ __forceinline__ float interp_scalar(const uint32_t (*raw)[texel_bytes(F) / 2], const float w[8])
+{
+    constexpr int H = texel_bytes(F) / 4;
+    float r = w[0] * decode_scalar<F>(raw[0]);
+#pragma unroll
+    for (int k = 1; k < 8; ++k) r = fmaf(w[k], decode_scalar<F>(raw[k >> 1] + (k & 1) * H), r);
+    return r;
+}
+template <int F>
+__device__ __forceinline__ float3 interp_normal(const uint32_t (*raw)[texel_bytes(F) / 2], const float w[8])
+{
+    constexpr int H = texel_bytes(F) / 4;
+    const float3 t0 = decode_normal<F>(raw[0]);
+    float n0 = w[0] * t0.x, n1 = w[0] * t0.y, n2 = w[0] * t0.z;
+#pragma unroll
+    for (int k = 1; k < 8; ++k) {
+        const float3 t = decode_normal<F>(raw[k >> 1] + (k & 1) * H);
+        n0 = fmaf(w[k], t.x, n0);
+        n1 = fmaf(w[k], t.y, n1);
+        n2 = fmaf(w[k], t.z, n2);
+    }
+    return make_float3(n0, n1, n2);
+}
+
 // The eight taps of a footprint, x fastest: tap i at (i&1, (i>>1)&1, i>>2).
 // Formats of 4 bytes and more load each x-adjacent pair with one load of
 // 2T bytes (texels are dword aligned); 1- and 2-byte formats load per tap.
@@ -398,25 +460,6 @@ __device__ __forceinline__ void tri_weights(float a, float b, float g, float w[8
 __device__ __forceinline__ void fp_weights(const Footprint &f, float w[8])
 {
     tri_weights(__builtin_amdgcn_fractf(f.u), __builtin_amdgcn_fractf(f.v), __builtin_amdgcn_fractf(f.w), w);
-}
-
-template <int F, int LB, bool BUF>
-__device__ __forceinline__ float4 sample4(const Src<BUF> &s, const Footprint &f)
-{
-    float w[8];
-    float4 t[8];
-    load_taps<F, LB, BUF>(s, f, t);
-    fp_weights(f, w);
-    float4 r;
-    r.x = w[0] * t[0].x; r.y = w[0] * t[0].y; r.z = w[0] * t[0].z; r.w = w[0] * t[0].w;
-#pragma unroll
-    for (int k = 1; k < 8; ++k) {
-        r.x = fmaf(w[k], t[k].x, r.x);
-        r.y = fmaf(w[k], t[k].y, r.y);
-        r.z = fmaf(w[k], t[k].z, r.z);
-        r.w = fmaf(w[k], t[k].w, r.w);
-    }
-    return r;
 }
 
 template <int F, int LB, bool BUF>
@@ -610,19 +653,19 @@ __device__ __forceinline__ float4 lut_global(const float4 *__restrict__ lut, int
 // correction 1 - exp(-alpha*step/ref), ambient + Lambert with the zero-vector
 // rule (a normal of zero or subnormal length has no direction and gives no
 // diffuse light), front-to-back composite.
-template <int NM>
-__device__ __forceinline__ void composite(const FrameConst &fc, const float4 &q, float n0, float n1, float n2,
-                                          float &C0, float &C1, float &C2, float &A)
+__device__ __forceinline__ float lambert(const FrameConst &fc, float n0, float n1, float n2)
+{
+    const float nn = n0 * n0 + n1 * n1 + n2 * n2;
+    const float dt = (n0 * fc.L[0] + n1 * fc.L[1] + n2 * fc.L[2]) * rsqrtf(nn);
+    return (fc.L_ok && nn >= 1.17549435e-38f) ? fmaxf(dt, 0.f) : 0.f;
+}
+
+__device__ __forceinline__ void composite_diff(const FrameConst &fc, const float4 &q, float diff, float &C0,
+                                               float &C1, float &C2, float &A)
 {
     // v_exp_f32 without the subnormal-result rescaling of exp2f: when
     // exp2(-x) is subnormal, 1 - exp2(-x) rounds to 1 either way
     const float a = 1.f - __builtin_amdgcn_exp2f(-(q.w * fc.alpha_k));
-    float diff = 0.f;
-    if constexpr (NM != NM_ZERO) {
-        const float nn = n0 * n0 + n1 * n1 + n2 * n2;
-        const float dt = (n0 * fc.L[0] + n1 * fc.L[1] + n2 * fc.L[2]) * rsqrtf(nn);
-        diff = (fc.L_ok && nn >= 1.17549435e-38f) ? fmaxf(dt, 0.f) : 0.f;
-    }
     const float lam = fmaf(fc.kd, diff, fc.ka);
     const float ta = (1.f - A) * a;
     const float tl = ta * lam;
@@ -630,6 +673,13 @@ __device__ __forceinline__ void composite(const FrameConst &fc, const float4 &q,
     C1 = fmaf(tl, q.y, C1);
     C2 = fmaf(tl, q.z, C2);
     A = A + ta;
+}
+
+template <int NM>
+__device__ __forceinline__ void composite(const FrameConst &fc, const float4 &q, float n0, float n1, float n2,
+                                          float &C0, float &C1, float &C2, float &A)
+{
+    composite_diff(fc, q, NM != NM_ZERO ? lambert(fc, n0, n1, n2) : 0.f, C0, C1, C2, A);
 }
 
 // K2 -- march and composite (volume.frag.glsl:84-121), one thread per ray.
@@ -660,7 +710,10 @@ __device__ __forceinline__ bool inbox(const FrameConst &fc, float wx, float wy, 
 // The march of one ray (volume.frag.glsl:84-121) from its setup record:
 // accumulated colour (C0, C1, C2), alpha A, and the number of composited
 // samples ns (COUNT).  s_val / s_dlt: the LUT and its differences in LDS.
-template <int SF, int NM, int LB, bool COUNT, bool SKIP, bool BUF>
+// GATE: interpolate and light the normal only for samples of non-zero
+// alpha (fewer instructions where the march is instruction-bound; the
+// result is identical either way).
+template <int SF, int NM, int LB, bool COUNT, bool SKIP, bool BUF, bool GATE = false>
 __device__ __forceinline__ void march_ray(const FrameConst &fc, const TexDesc &ts, const TexDesc &tn,
                                           const float4 *__restrict__ lut_g, int lut_n, const float4 *s_val,
                                           const float4 *s_dlt, const RayRec &r, const uint8_t *__restrict__ skip,
@@ -710,17 +763,35 @@ __device__ __forceinline__ void march_ray(const FrameConst &fc, const TexDesc &t
         const Src<BUF> src(ts);
         // one sample's contribution from its footprint
         auto shade = [&](const Footprint &f) {
-            float rho, n0, n1 = 0.f, n2 = 0.f;
             if constexpr (NM == NM_SAME) {
-                const float4 t = sample4<SF, LB, BUF>(src, f);
-                rho = t.w * u8k;
-                n0 = t.x; n1 = t.y; n2 = t.z;
+                constexpr int T = texel_bytes(SF);
+                uint32_t raw[4][T / 2];
+                load_pairs<SF, LB, BUF>(src, f, raw);
+                float w[8];
+                fp_weights(f, w);
+                const float rho = interp_scalar<SF>(raw, w) * u8k;
+                float diff = 0.f;
+                if constexpr (GATE) {
+                    // Normals only for a non-zero alpha: with alpha 0 the
+                    // sample adds (1-A)*0*lambda = 0 for any finite lambda
+                    // (the Lambert term is 0 or fmaxf(dot, 0)), so skipping
+                    // the normal changes nothing; the texel loads are shared.
+                    const float4 q = lut_sample(s_val, s_dlt, fn1, fn, rho);
+                    if (q.w != 0.f) {
+                        const float3 n = interp_normal<SF>(raw, w);
+                        diff = lambert(fc, n.x, n.y, n.z);
+                    }
+                    composite_diff(fc, q, diff, C0, C1, C2, A);
+                } else {
+                    const float3 n = interp_normal<SF>(raw, w);
+                    const float4 q = lut_sample(s_val, s_dlt, fn1, fn, rho);
+                    composite_diff(fc, q, lambert(fc, n.x, n.y, n.z), C0, C1, C2, A);
+                }
             } else {
-                rho = sample1<SF, LB, BUF>(src, f) * u8k;
-                n0 = NM == NM_ALIAS ? rho : 0.f;
+                const float rho = sample1<SF, LB, BUF>(src, f) * u8k;
+                const float4 q = lut_sample(s_val, s_dlt, fn1, fn, rho);
+                composite<NM>(fc, q, NM == NM_ALIAS ? rho : 0.f, 0.f, 0.f, C0, C1, C2, A);
             }
-            const float4 q = lut_sample(s_val, s_dlt, fn1, fn, rho);
-            composite<NM>(fc, q, n0, n1, n2, C0, C1, C2, A);
         };
         if constexpr (!SKIP) {
             // the in-box run
@@ -1184,8 +1255,8 @@ k_batch(const char *__restrict__ block, int cls, uint32_t *__restrict__ counter,
             const RayRec r = ray_setup(R.fc, px, py);
             float C0 = 0.f, C1 = 0.f, C2 = 0.f, A = 0.f;
             uint32_t ns = 0;
-            march_ray<SF, NM, LB, COUNT, true, BUF>(R.fc, R.ts, R.ts, R.lut, R.lut_n, s_val, s_dlt, r, R.flags,
-                                                    C0, C1, C2, A, ns);
+            march_ray<SF, NM, LB, COUNT, true, BUF, true>(R.fc, R.ts, R.ts, R.lut, R.lut_n, s_val, s_dlt, r,
+                                                          R.flags, C0, C1, C2, A, ns);
             const int pix = py * R.fc.W + px;
             st_global(R.out + pix, blend_store(R.fc, C0, C1, C2, A));
             if constexpr (COUNT) st_global(R.counts + pix, ns);
