@@ -684,10 +684,16 @@ __device__ __forceinline__ void composite(const FrameConst &fc, const float4 &q,
 
 // K2 -- march and composite (volume.frag.glsl:84-121), one thread per ray.
 // A 256-thread block covers 16x16 pixels; each wave64 an 8x8 tile.  Blocks
-// are spread so that each of the 8 XCDs gets a contiguous band of tiles (its
-// own L2 sees a compact part of the volume).  A wave runs while any of its
-// rays is live; a ray stops at its iteration bound (max_steps at most), when
-// its alpha reaches tau, or when it leaves the box.
+// are dispatched round-robin over the 8 XCDs (block b on XCD (b + c) mod 8,
+// c varying from launch to launch), and each XCD runs its blocks in order.
+// The block -> tile mapping deals 16-pixel block rows to the XCDs in turn
+// (XCD of b & 7 takes rows b & 7, 8 + (b & 7), ...): the work per row varies
+// strongly over the image (where the rays cross dense matter), so
+// contiguous bands per XCD left some XCDs idle for half the frame, while
+// interleaved rows give every XCD an even share, and its L2 still sees whole
+// rows.  Rows past the last multiple of 8 keep the plain order.  A wave runs
+// while any of its rays is live; a ray stops at its iteration bound
+// (max_steps at most), when its alpha reaches tau, or when it leaves the box.
 //
 // In-box test (:88-92): fl((p - bmin)/ext) lies in [0, 1] exactly when
 // fl(p - bmin) lies in [0, ext] (for ext > 0 the quotient of a larger
@@ -859,8 +865,12 @@ k_march(const FrameConst fc, const TexDesc ts, const TexDesc tn, const float4 *_
         __syncthreads();
     }
     const int b = blockIdx.x;
-    const int per = nblocks >> 3, full = per << 3;
-    const int logical = b < full ? (b & 7) * per + (b >> 3) : b;
+    const int head = ((nblocks / ntx) >> 3 << 3) * ntx;   // blocks of the first multiple-of-8 rows
+    int logical = b;
+    if (b < head) {
+        const int j = b >> 3;   // j-th block of its XCD
+        logical = ((j / ntx) * 8 + (b & 7)) * ntx + j % ntx;
+    }
     const int tx = logical % ntx, ty = logical / ntx;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int px = tx * 16 + (wv & 1) * 8 + (lane & 7);

@@ -179,3 +179,20 @@ def test_vti_example_files(hip, name):
     ref = oracle_render(sc)
     assert_parity(img, ref["rgba8"], label=sc.name)
     assert ref["rgba8"][..., 3].max() > 0
+
+
+@pytest.mark.parametrize("wh", [(200, 170), (168, 136)])
+def test_non_square_ragged_frames(hip, wh):
+    """Frames whose size is not a multiple of the 16-pixel block, with block
+    rows both a multiple of 8 and not (the march deals whole groups of 8
+    block rows to the XCDs and keeps the rest in plain order): every pixel
+    is rendered, to the oracle's bar, with the oracle's sample counts."""
+    import dataclasses
+
+    w, h = wh
+    sc = dataclasses.replace(scenes.c0(size=48, image=96), width=w, height=h)
+    img, cnt = render_hip(sc, counts=True)
+    ref = oracle_render(sc, mode="f32")
+    assert_parity(img, oracle_render(sc)["rgba8"], label=f"{w}x{h}")
+    diff = np.abs(cnt.astype(np.int64) - ref["counts"].astype(np.int64))
+    assert (diff == 0).mean() > 0.999 and ref["counts"].sum() > 0
