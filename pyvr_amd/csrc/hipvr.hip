@@ -1067,8 +1067,7 @@ static int batch_launch(const char *blk, int cls, uint32_t *ctr, int lut_cap, in
             hipSuccess ||
         per < 1)
         per = 1;
-    hipLaunchKernelGGL((k_batch<SF, NM, LB, COUNT, TS, BUF>), dim3(per * ncu), dim3(256), lds, st, blk, cls, ctr,
-                       lut_cap);
+    hipLaunchKernelGGL((k_batch<SF, NM, LB, COUNT, TS, BUF>), dim3(per * ncu), dim3(256), lds, st, blk, cls, ctr);
     return hipGetLastError() == hipSuccess ? HVR_OK : HVR_EDEVICE;
 }
 

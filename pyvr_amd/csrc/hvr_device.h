@@ -227,13 +227,14 @@ struct Footprint {
 template <int T, int LB, bool BUF>
 __device__ __forceinline__ Footprint footprint(const TexDesc &t, float w0, float w1, float w2)
 {
+    // u, v, w >= 0 for in-box positions (and the 0 the callers pass for
+    // others), so truncation is floor
     const float u = fmaf(w0, t.k0, 0.5f), v = fmaf(w1, t.k1, 0.5f), w = fmaf(w2, t.k2, 0.5f);
-    const float fu = floorf(u), fv = floorf(v), fw = floorf(w);
     Footprint f;
     f.u = u;
     f.v = v;
     f.w = w;
-    const uint32_t X = (uint32_t)(int)fu, Y = (uint32_t)(int)fv, Z = (uint32_t)(int)fw;
+    const uint32_t X = (uint32_t)u, Y = (uint32_t)v, Z = (uint32_t)w;
     constexpr uint32_t M = (1u << LB) - 1u, E = (1u << LB) + 1u;
     const uint32_t local = ((Z & M) * E + (Y & M)) * E + (X & M);
     uint32_t brick;
@@ -621,17 +622,17 @@ enum NormalMode : int {
 };
 
 // LUT lookup at (rho, 0.5) with u = rho*N - 1/2, clamp to edge
-// (manager.py:163-181; volume.frag.glsl:96).  `val` holds the N entries and
-// `dlt` the differences to the next entry (0 after the last), so that
-// clamping u to [0, N-1] gives both edge cases and one fma per channel.
-template <typename P>
-__device__ __forceinline__ float4 lut_sample(P val, P dlt, float fn1, float fn, float rho)
+// (manager.py:163-181; volume.frag.glsl:96).  `lut` holds, per entry i, the
+// pair (entry i, entry i+1 - entry i) (difference 0 after the last), so that
+// clamping u to [0, N-1] gives both edge cases and one fma per channel.  u
+// >= 0 after the clamp (fmaxf also maps a NaN to 0): truncation is floor and
+// v_fract_f32 gives u - floor(u) exactly.
+__device__ __forceinline__ float4 lut_sample(const float4 *lut, float fn1, float fn, float rho)
 {
     const float u = fminf(fmaxf(fmaf(rho, fn, -0.5f), 0.f), fn1);
-    const float fu = floorf(u);
-    const float a = u - fu;
-    const int i = (int)fu;
-    const float4 v = val[i], d = dlt[i];
+    const float a = __builtin_amdgcn_fractf(u);
+    const int i = (int)u;
+    const float4 v = lut[2 * i], d = lut[2 * i + 1];
     return make_float4(fmaf(a, d.x, v.x), fmaf(a, d.y, v.y), fmaf(a, d.z, v.z), fmaf(a, d.w, v.w));
 }
 
@@ -655,8 +656,8 @@ __device__ __forceinline__ float4 lut_global(const float4 *__restrict__ lut, int
 // diffuse light), front-to-back composite.
 __device__ __forceinline__ float lambert(const FrameConst &fc, float n0, float n1, float n2)
 {
-    const float nn = n0 * n0 + n1 * n1 + n2 * n2;
-    const float dt = (n0 * fc.L[0] + n1 * fc.L[1] + n2 * fc.L[2]) * rsqrtf(nn);
+    const float nn = fmaf(n0, n0, fmaf(n1, n1, n2 * n2));
+    const float dt = fmaf(n0, fc.L[0], fmaf(n1, fc.L[1], n2 * fc.L[2])) * rsqrtf(nn);
     return (fc.L_ok && nn >= 1.17549435e-38f) ? fmaxf(dt, 0.f) : 0.f;
 }
 
@@ -715,14 +716,14 @@ __device__ __forceinline__ bool inbox(const FrameConst &fc, float wx, float wy, 
 
 // The march of one ray (volume.frag.glsl:84-121) from its setup record:
 // accumulated colour (C0, C1, C2), alpha A, and the number of composited
-// samples ns (COUNT).  s_val / s_dlt: the LUT and its differences in LDS.
+// samples ns (COUNT).  s_lut: the LUT with its differences in LDS (lut_sample).
 // GATE: interpolate and light the normal only for samples of non-zero
 // alpha (fewer instructions where the march is instruction-bound; the
 // result is identical either way).
 template <int SF, int NM, int LB, bool COUNT, bool SKIP, bool BUF, bool GATE = false>
 __device__ __forceinline__ void march_ray(const FrameConst &fc, const TexDesc &ts, const TexDesc &tn,
-                                          const float4 *__restrict__ lut_g, int lut_n, const float4 *s_val,
-                                          const float4 *s_dlt, const RayRec &r, const uint8_t *__restrict__ skip,
+                                          const float4 *__restrict__ lut_g, int lut_n, const float4 *s_lut,
+                                          const RayRec &r, const uint8_t *__restrict__ skip,
                                           float &C0, float &C1, float &C2, float &A, uint32_t &ns)
 {
     float x = r.p0x, y = r.p0y, z = r.p0z;
@@ -759,7 +760,7 @@ __device__ __forceinline__ void march_ray(const FrameConst &fc, const TexDesc &t
                 }
             }
             const float4 q = NM == NM_ANY ? lut_global(lut_g, lut_n, fn1, fn, rho)
-                                          : lut_sample(s_val, s_dlt, fn1, fn, rho);
+                                          : lut_sample(s_lut, fn1, fn, rho);
             composite<NM>(fc, q, n0, n1, n2, C0, C1, C2, A);
             x = __fadd_rn(x, r.dsx);
             y = __fadd_rn(y, r.dsy);
@@ -782,7 +783,7 @@ __device__ __forceinline__ void march_ray(const FrameConst &fc, const TexDesc &t
                     // sample adds (1-A)*0*lambda = 0 for any finite lambda
                     // (the Lambert term is 0 or fmaxf(dot, 0)), so skipping
                     // the normal changes nothing; the texel loads are shared.
-                    const float4 q = lut_sample(s_val, s_dlt, fn1, fn, rho);
+                    const float4 q = lut_sample(s_lut, fn1, fn, rho);
                     if (q.w != 0.f) {
                         const float3 n = interp_normal<SF>(raw, w);
                         diff = lambert(fc, n.x, n.y, n.z);
@@ -790,12 +791,12 @@ __device__ __forceinline__ void march_ray(const FrameConst &fc, const TexDesc &t
                     composite_diff(fc, q, diff, C0, C1, C2, A);
                 } else {
                     const float3 n = interp_normal<SF>(raw, w);
-                    const float4 q = lut_sample(s_val, s_dlt, fn1, fn, rho);
+                    const float4 q = lut_sample(s_lut, fn1, fn, rho);
                     composite_diff(fc, q, lambert(fc, n.x, n.y, n.z), C0, C1, C2, A);
                 }
             } else {
                 const float rho = sample1<SF, LB, BUF>(src, f) * u8k;
-                const float4 q = lut_sample(s_val, s_dlt, fn1, fn, rho);
+                const float4 q = lut_sample(s_lut, fn1, fn, rho);
                 composite<NM>(fc, q, NM == NM_ALIAS ? rho : 0.f, 0.f, 0.f, C0, C1, C2, A);
             }
         };
@@ -836,9 +837,9 @@ __device__ __forceinline__ void march_ray(const FrameConst &fc, const TexDesc &t
     }
 }
 
-// Load the LUT of n entries and the differences to the next entry (0 after
-// the last) into LDS.
-__device__ __forceinline__ void load_lut_lds(const float4 *__restrict__ lut_g, int n, float4 *s_val, float4 *s_dlt)
+// Load the LUT of n entries, each followed by its difference to the next
+// entry (0 after the last), into LDS.
+__device__ __forceinline__ void load_lut_lds(const float4 *__restrict__ lut_g, int n, float4 *s_lut)
 {
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const float4 v = ld_global(lut_g + i);
@@ -847,8 +848,8 @@ __device__ __forceinline__ void load_lut_lds(const float4 *__restrict__ lut_g, i
             const float4 nx = ld_global(lut_g + i + 1);
             d = make_float4(nx.x - v.x, nx.y - v.y, nx.z - v.z, nx.w - v.w);
         }
-        s_val[i] = v;
-        s_dlt[i] = d;
+        s_lut[2 * i] = v;
+        s_lut[2 * i + 1] = d;
     }
 }
 
@@ -859,9 +860,8 @@ k_march(const FrameConst fc, const TexDesc ts, const TexDesc tn, const float4 *_
         uint32_t *__restrict__ counts, const uint8_t *__restrict__ skip, int ntx, int nblocks)
 {
     extern __shared__ float4 s_lut[];
-    float4 *s_val = s_lut, *s_dlt = s_lut + lut_n;
     if constexpr (NM != NM_ANY) {
-        load_lut_lds(lut_g, lut_n, s_val, s_dlt);
+        load_lut_lds(lut_g, lut_n, s_lut);
         __syncthreads();
     }
     const int b = blockIdx.x;
@@ -881,7 +881,7 @@ k_march(const FrameConst fc, const TexDesc ts, const TexDesc tn, const float4 *_
 
     float C0 = 0.f, C1 = 0.f, C2 = 0.f, A = 0.f;
     uint32_t ns = 0;
-    march_ray<SF, NM, LB, COUNT, SKIP, BUF>(fc, ts, tn, lut_g, lut_n, s_val, s_dlt, r, skip, C0, C1, C2, A, ns);
+    march_ray<SF, NM, LB, COUNT, SKIP, BUF>(fc, ts, tn, lut_g, lut_n, s_lut, r, skip, C0, C1, C2, A, ns);
     out[pix] = blend_store(fc, C0, C1, C2, A);
     if constexpr (COUNT) counts[pix] = ns;
 }
@@ -1218,7 +1218,7 @@ struct TileShape<1> {
 
 template <int SF, int NM, int LB, bool COUNT, int TS, bool BUF>
 __global__ void __launch_bounds__(256)
-k_batch(const char *__restrict__ block, int cls, uint32_t *__restrict__ counter, int lut_cap)
+k_batch(const char *__restrict__ block, int cls, uint32_t *__restrict__ counter)
 {
     extern __shared__ float4 s_lut[];
     __shared__ int s_item[2];
@@ -1228,7 +1228,6 @@ k_batch(const char *__restrict__ block, int cls, uint32_t *__restrict__ counter,
     if (nreq <= 0) return;
     const ReqDesc *__restrict__ reqs = (const ReqDesc *)(block + hdr->cls[cls].desc_offset);
     const int total = reqs[nreq - 1].tile_end;
-    float4 *s_val = s_lut, *s_dlt = s_lut + lut_cap;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     int cur = -1;
     for (;;) {
@@ -1253,7 +1252,7 @@ k_batch(const char *__restrict__ block, int cls, uint32_t *__restrict__ counter,
         if (t >= total) break;
         const ReqDesc &R = reqs[q];
         if (q != cur) {
-            load_lut_lds(R.lut, R.lut_n, s_val, s_dlt);
+            load_lut_lds(R.lut, R.lut_n, s_lut);
             __syncthreads();
             cur = q;
         }
@@ -1265,7 +1264,7 @@ k_batch(const char *__restrict__ block, int cls, uint32_t *__restrict__ counter,
             const RayRec r = ray_setup(R.fc, px, py);
             float C0 = 0.f, C1 = 0.f, C2 = 0.f, A = 0.f;
             uint32_t ns = 0;
-            march_ray<SF, NM, LB, COUNT, true, BUF, true>(R.fc, R.ts, R.ts, R.lut, R.lut_n, s_val, s_dlt, r,
+            march_ray<SF, NM, LB, COUNT, true, BUF, true>(R.fc, R.ts, R.ts, R.lut, R.lut_n, s_lut, r,
                                                           R.flags, C0, C1, C2, A, ns);
             const int pix = py * R.fc.W + px;
             st_global(R.out + pix, blend_store(R.fc, C0, C1, C2, A));
