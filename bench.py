@@ -41,7 +41,7 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 SCALAR_BYTES = {"f32": 4, "f16": 2, "u8": 1}
-NORMAL_BYTES = {"f32x3": 12, "sn8x3": 3, "none": 0}
+NORMAL_BYTES = {"f32x3": 12, "sn8x3": 3, "f16x3": 6, "none": 0}
 METRIC = "ray samples/s + fps at 512^3/1024^2; p50/p99 frame latency vs plain kernel"
 
 
@@ -406,7 +406,7 @@ def main():
     ap.add_argument("--image", type=int, default=0)
     ap.add_argument("--cameras", type=int, default=24)
     ap.add_argument("--scalar-format", default="f16", choices=["f32", "f16", "u8"])
-    ap.add_argument("--normal-format", default="sn8x3", choices=["f32x3", "sn8x3"])
+    ap.add_argument("--normal-format", default="sn8x3", choices=["f32x3", "sn8x3", "f16x3"])
     ap.add_argument("--brick-log2", type=int, default=-1)
     ap.add_argument("--skip", choices=("auto", "on", "off"), default="auto",
                     help="transparent-brick skipping mode (result is identical in every mode)")

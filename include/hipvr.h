@@ -47,6 +47,8 @@ extern "C" {
 #define HVR_NORMAL_NONE 0
 #define HVR_NORMAL_F32X3 1 /* 3 x float32                                   */
 #define HVR_NORMAL_SN8X3 2 /* SNORM8: round(127 * clamp(n, -1, 1)) per axis */
+#define HVR_NORMAL_F16X3 3 /* 3 x IEEE half, round to nearest even; scalar   */
+                           /* NONE or F16 only (8-byte texels)               */
 
 /* which field of a volume a sampler reads (see hvr_bind) */
 #define HVR_ROLE_SCALAR 0  /* the scalar field                               */

@@ -321,6 +321,7 @@ static void launch_march(hvr_ctx *c, const FrameConst &fc)
                 HVR_CASE(TF_C_F32_SN8)
                 HVR_CASE(TF_C_F16_SN8)
                 HVR_CASE(TF_C_U8_SN8)
+                HVR_CASE(TF_C_F16_F16)
             default: break;
             }
         }
@@ -391,6 +392,7 @@ static int compute_range(hvr_volume *v)
         HVR_RG(TF_C_F32_SN8)
         HVR_RG(TF_C_F16_SN8)
         HVR_RG(TF_C_U8_SN8)
+        HVR_RG(TF_C_F16_F16)
     default: return HVR_OK;
     }
 #undef HVR_RG
@@ -533,6 +535,12 @@ static int texel_format(int sf, int nf)
         if (sf == HVR_SCALAR_F32) return TF_C_F32_SN8;
         if (sf == HVR_SCALAR_F16) return TF_C_F16_SN8;
         if (sf == HVR_SCALAR_U8) return TF_C_U8_SN8;
+        return -1;
+    }
+    if (nf == HVR_NORMAL_F16X3) {
+        // 8-byte texels; FP32 and UNORM8 scalars are not paired with it
+        if (sf == HVR_SCALAR_NONE) return TF_N_F16;
+        if (sf == HVR_SCALAR_F16) return TF_C_F16_F16;
     }
     return -1;
 }
@@ -625,6 +633,8 @@ static int upload_part(hvr_volume *v, const float *data, int part)
             HVR_UP(TF_C_F32_SN8)
             HVR_UP(TF_C_F16_SN8)
             HVR_UP(TF_C_U8_SN8)
+            HVR_UP(TF_C_F16_F16)
+            HVR_UP(TF_N_F16)
         default: return fail(HVR_EINVAL, "bad texel format");
         }
 #undef HVR_UP
@@ -662,6 +672,7 @@ static int volume_from_field(hvr_volume *v, const float *d_field, bool normals)
         HVR_UPF(TF_C_F32_SN8)
         HVR_UPF(TF_C_F16_SN8)
         HVR_UPF(TF_C_U8_SN8)
+        HVR_UPF(TF_C_F16_F16)
     default: return fail(HVR_EINVAL, "the volume has no scalar field");
     }
 #undef HVR_UPF
@@ -1105,6 +1116,7 @@ static BatchLaunchFn batch_fn_c(int fmt, int nm, int lb, int ts)
         HVR_C(TF_C_F32_SN8)
         HVR_C(TF_C_F16_SN8)
         HVR_C(TF_C_U8_SN8)
+        HVR_C(TF_C_F16_F16)
     default: return nullptr;
     }
 #undef HVR_S

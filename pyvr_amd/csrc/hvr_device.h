@@ -23,7 +23,10 @@ namespace hvr {
 // 1-channel texture; for normal-only texels rho reads nx (the .r channel).
 // In the fast paths SNORM8 normals stay raw integers (the normal is
 // normalised afterwards, so the 1/127 scale cancels) and UNORM8 scalars are
-// scaled by 1/255 after interpolation (filtering is linear).
+// scaled by 1/255 after interpolation (filtering is linear).  FP16 scalars
+// and normals are interpolated straight from their halves (v_fma_mix_f32:
+// float32 accumulation of exactly converted halves), with no conversion
+// instructions.
 enum TexFmt : int {
     TF_NONE = 0,
     TF_S_F32 = 1,      // float                                    4 B
@@ -37,7 +40,9 @@ enum TexFmt : int {
     TF_C_F32_SN8 = 9,  // u32x2 (float rho, sn8 xyz)               8 B
     TF_C_F16_SN8 = 10, // u32x2 (half rho, sn8 xyz)                8 B
     TF_C_U8_SN8 = 11,  // u32 sn8 xyz | u8 rho << 24               4 B
-    TF_COUNT = 12
+    TF_C_F16_F16 = 12, // u32x2 (half rho | half nx << 16, half ny | half nz << 16)  8 B
+    TF_N_F16 = 13,     // u32x2 (half nx | half ny << 16, half nz)  8 B
+    TF_COUNT = 14
 };
 
 __host__ __device__ constexpr int texel_bytes(int f)
@@ -45,15 +50,16 @@ __host__ __device__ constexpr int texel_bytes(int f)
     return f == TF_S_F32 ? 4 : f == TF_S_F16 ? 2 : f == TF_S_U8 ? 1
          : f == TF_N_F32 ? 16 : f == TF_N_SN8 ? 4
          : (f == TF_C_F32_F32 || f == TF_C_F16_F32 || f == TF_C_U8_F32) ? 16
-         : (f == TF_C_F32_SN8 || f == TF_C_F16_SN8) ? 8 : f == TF_C_U8_SN8 ? 4 : 0;
+         : (f == TF_C_F32_SN8 || f == TF_C_F16_SN8) ? 8 : f == TF_C_U8_SN8 ? 4
+         : (f == TF_C_F16_F16 || f == TF_N_F16) ? 8 : 0;
 }
 __host__ __device__ constexpr bool fmt_has_scalar(int f)
 {
-    return f == TF_S_F32 || f == TF_S_F16 || f == TF_S_U8 || f >= TF_C_F32_F32;
+    return f == TF_S_F32 || f == TF_S_F16 || f == TF_S_U8 || (f >= TF_C_F32_F32 && f <= TF_C_F16_F16);
 }
 __host__ __device__ constexpr bool fmt_has_normal(int f)
 {
-    return f == TF_N_F32 || f == TF_N_SN8 || f >= TF_C_F32_F32;
+    return f == TF_N_F32 || f == TF_N_SN8 || f == TF_N_F16 || f >= TF_C_F32_F32;
 }
 __host__ __device__ constexpr bool fmt_sn8_normal(int f)
 {
@@ -121,6 +127,8 @@ __device__ __forceinline__ void st_global(uchar4 *p, const uchar4 &v)
 }
 
 __device__ __forceinline__ float sn8f(uint32_t w, int sh) { return (float)(int8_t)((w >> sh) & 0xffu); }
+// the half in the low 16 bits of w
+__device__ __forceinline__ float h16f(uint32_t w) { return __half2float(__ushort_as_half((unsigned short)(w & 0xffffu))); }
 
 // Decode the texel at byte address p to (nx, ny, nz, rho) (raw SNORM8/UNORM8).
 template <int F>
@@ -161,6 +169,13 @@ __device__ __forceinline__ float4 fetch4(const char *p)
     } else if constexpr (F == TF_C_U8_SN8) {
         const uint32_t w = *(const uint32_t *)p;
         return make_float4(sn8f(w, 0), sn8f(w, 8), sn8f(w, 16), (float)(w >> 24));
+    } else if constexpr (F == TF_C_F16_F16) {
+        const uint2 t = *(const uint2 *)p;
+        return make_float4(h16f(t.x >> 16), h16f(t.y), h16f(t.y >> 16), h16f(t.x));
+    } else if constexpr (F == TF_N_F16) {
+        const uint2 t = *(const uint2 *)p;
+        const float x = h16f(t.x);
+        return make_float4(x, h16f(t.x >> 16), h16f(t.y), x);
     } else {
         return make_float4(0.f, 0.f, 0.f, 0.f);
     }
@@ -192,6 +207,8 @@ __device__ __forceinline__ float4 fetch_any(int fmt, const char *p)
     case TF_C_F32_SN8: t = fetch4<TF_C_F32_SN8>(p); break;
     case TF_C_F16_SN8: t = fetch4<TF_C_F16_SN8>(p); break;
     case TF_C_U8_SN8: t = fetch4<TF_C_U8_SN8>(p); break;
+    case TF_C_F16_F16: t = fetch4<TF_C_F16_F16>(p); break;
+    case TF_N_F16: t = fetch4<TF_N_F16>(p); break;
     default: return make_float4(0.f, 0.f, 0.f, 0.f);
     }
     const float k127 = 1.0f / 127.0f, k255 = 1.0f / 255.0f;
@@ -343,6 +360,11 @@ __device__ __forceinline__ float4 decode_words(const uint32_t *w)
                            __half2float(__ushort_as_half((unsigned short)(w[0] & 0xffffu))));
     } else if constexpr (F == TF_C_U8_SN8) {
         return make_float4(sn8f(w[0], 0), sn8f(w[0], 8), sn8f(w[0], 16), (float)(w[0] >> 24));
+    } else if constexpr (F == TF_C_F16_F16) {
+        return make_float4(h16f(w[0] >> 16), h16f(w[1]), h16f(w[1] >> 16), h16f(w[0]));
+    } else if constexpr (F == TF_N_F16) {
+        const float x = h16f(w[0]);
+        return make_float4(x, h16f(w[0] >> 16), h16f(w[1]), x);
     } else {
         return make_float4(0.f, 0.f, 0.f, 0.f);
     }
@@ -358,6 +380,7 @@ __device__ __forceinline__ float decode_scalar(const uint32_t *w)
     else if constexpr (F == TF_C_F32_SN8) return __uint_as_float(w[0]);
     else if constexpr (F == TF_C_F16_SN8) return __half2float(__ushort_as_half((unsigned short)(w[0] & 0xffffu)));
     else if constexpr (F == TF_C_U8_SN8) return (float)(w[0] >> 24);
+    else if constexpr (F == TF_C_F16_F16) return h16f(w[0]);
     else return 0.f;
 }
 template <int F>
@@ -369,6 +392,8 @@ __device__ __forceinline__ float3 decode_normal(const uint32_t *w)
         return make_float3(sn8f(w[1], 0), sn8f(w[1], 8), sn8f(w[1], 16));
     else if constexpr (F == TF_C_U8_SN8)
         return make_float3(sn8f(w[0], 0), sn8f(w[0], 8), sn8f(w[0], 16));
+    else if constexpr (F == TF_C_F16_F16)
+        return make_float3(h16f(w[0] >> 16), h16f(w[1]), h16f(w[1] >> 16));
     else return make_float3(0.f, 0.f, 0.f);
 }
 
@@ -389,7 +414,8 @@ template <int F>
 __device__ __forceinline__ float interp_scalar(const uint32_t (*raw)[texel_bytes(F) / 2], const float w[8])
 {
     constexpr int H = texel_bytes(F) / 4;
-    float r = w[0] * decode_scalar<F>(raw[0]);
+    // an fma from 0 (not a product) lets a half operand fold into v_fma_mix
+    float r = fmaf(w[0], decode_scalar<F>(raw[0]), 0.f);
 #pragma unroll
     for (int k = 1; k < 8; ++k) r = fmaf(w[k], decode_scalar<F>(raw[k >> 1] + (k & 1) * H), r);
     return r;
@@ -399,7 +425,7 @@ __device__ __forceinline__ float3 interp_normal(const uint32_t (*raw)[texel_byte
 {
     constexpr int H = texel_bytes(F) / 4;
     const float3 t0 = decode_normal<F>(raw[0]);
-    float n0 = w[0] * t0.x, n1 = w[0] * t0.y, n2 = w[0] * t0.z;
+    float n0 = fmaf(w[0], t0.x, 0.f), n1 = fmaf(w[0], t0.y, 0.f), n2 = fmaf(w[0], t0.z, 0.f);
 #pragma unroll
     for (int k = 1; k < 8; ++k) {
         const float3 t = decode_normal<F>(raw[k >> 1] + (k & 1) * H);
@@ -1011,6 +1037,7 @@ __device__ __forceinline__ void store_part(char *p, int part, const float *val)
         else if constexpr (F == TF_C_F32_SN8) ((float *)p)[0] = v;
         else if constexpr (F == TF_C_F16_SN8) ((uint32_t *)p)[0] = (uint32_t)__half_as_ushort(__float2half_rn(v));
         else if constexpr (F == TF_C_U8_SN8) ((uint8_t *)p)[3] = (uint8_t)enc_u8(v);
+        else if constexpr (F == TF_C_F16_F16) *(__half *)p = __float2half_rn(v);
     } else {
         const float nx = val[0], ny = val[1], nz = val[2];
         const uint32_t packed = enc_sn8(nx) | (enc_sn8(ny) << 8) | (enc_sn8(nz) << 16);
@@ -1026,6 +1053,13 @@ __device__ __forceinline__ void store_part(char *p, int part, const float *val)
             uint8_t *q = (uint8_t *)p;
             q[0] = (uint8_t)(packed & 0xffu); q[1] = (uint8_t)((packed >> 8) & 0xffu);
             q[2] = (uint8_t)((packed >> 16) & 0xffu);
+        } else if constexpr (F == TF_C_F16_F16) {
+            __half *q = (__half *)p;   // halves 1..3; half 0 is the scalar
+            q[1] = __float2half_rn(nx); q[2] = __float2half_rn(ny); q[3] = __float2half_rn(nz);
+        } else if constexpr (F == TF_N_F16) {
+            __half *q = (__half *)p;
+            q[0] = __float2half_rn(nx); q[1] = __float2half_rn(ny); q[2] = __float2half_rn(nz);
+            q[3] = __float2half_rn(0.f);
         }
     }
 }

@@ -14,7 +14,18 @@ LIB_PATH = os.environ.get("HVR_LIBRARY") or os.path.join(_PKG, "libhipvr.so")
 
 HVR_OK = 0
 HVR_SCALAR = {"none": 0, "f32": 1, "f16": 2, "u8": 3}
-HVR_NORMAL = {"none": 0, "f32x3": 1, "sn8x3": 2}
+HVR_NORMAL = {"none": 0, "f32x3": 1, "sn8x3": 2, "f16x3": 3}
+
+
+def check_formats(scalar_format: str, normal_format: str) -> None:
+    """``ValueError`` unless the library stores this pair (canonical names;
+    "none" for an absent field): FP16x3 normals pair with FP16 scalars or
+    none, the other normal formats with every scalar format."""
+    sf, nf = scalar_format, normal_format
+    if (sf not in HVR_SCALAR or nf not in HVR_NORMAL or (sf == "none" and nf == "none")
+            or (nf == "f16x3" and sf not in ("none", "f16"))):
+        raise ValueError(f"unsupported formats scalar={sf!r} normal={nf!r}"
+                         + (" (f16x3 normals pair with f16 scalars)" if nf == "f16x3" else ""))
 HVR_ROLE_SCALAR = 0
 HVR_ROLE_NORMAL = 1
 

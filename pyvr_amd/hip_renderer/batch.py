@@ -115,6 +115,7 @@ class ResidentVolume:
         nf = _FORMAT_ALIASES.get(normal_format) if self.has_normals else "none"
         if sf not in _lib.HVR_SCALAR or sf == "none" or nf not in _lib.HVR_NORMAL:
             raise ValueError(f"unsupported formats scalar={scalar_format!r} normal={normal_format!r}")
+        _lib.check_formats(sf, nf)
         self.volume = volume
         shape = tuple(volume.shape) if generated else volume.data.shape
         self.device = _DeviceVolume(L, ctx, shape, sf, nf)
@@ -165,7 +166,8 @@ class BatchRenderer:
     """Renders many requests per launch on one GPU.
 
     ``scalar_format`` / ``normal_format`` select the device storage of the
-    volumes it uploads ("f32"/"f16"/"u8", "f32x3"/"sn8x3"); ``brick_log2``
+    volumes it uploads ("f32"/"f16"/"u8", "f32x3"/"sn8x3", or "f16x3"
+    with "f16"); ``brick_log2``
     the brick edge (3: 8^3, 4: 16^3) and ``tile_shape`` the ray tile of a
     wavefront ("8x8" or "16x4") -- the choices the autotuner makes.
 

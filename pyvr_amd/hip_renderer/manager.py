@@ -148,7 +148,7 @@ class _TextureLUT:
 _FORMAT_ALIASES = {
     None: None, "f32": "f32", "float32": "f32", "f16": "f16", "float16": "f16", "half": "f16",
     "u8": "u8", "unorm8": "u8", "uint8": "u8",
-    "f32x3": "f32x3", "sn8x3": "sn8x3", "snorm8x3": "sn8x3",
+    "f32x3": "f32x3", "sn8x3": "sn8x3", "snorm8x3": "sn8x3", "f16x3": "f16x3", "float16x3": "f16x3",
 }
 
 
@@ -206,7 +206,7 @@ class HipManager:
         """Upload a 3-D scalar field (reference manager.py:77-108) and return
         its texture unit.  Any dtype is converted to float32 first.
         ``scalar_format`` selects the device storage ("f32", "f16", "u8");
-        ``normal_format`` ("f32x3", "sn8x3") reserves an interleaved normal
+        ``normal_format`` ("f32x3", "sn8x3", "f16x3" with "f16") reserves an interleaved normal
         field that the next ``create_normal_texture`` of the same shape fills,
         or, with ``device_normals``, that the device computes from the field
         (``create_device_normal_texture`` then binds it)."""
@@ -214,8 +214,9 @@ class HipManager:
             raise ValueError("Volume data must be 3D")
         sf = _FORMAT_ALIASES.get(scalar_format)
         nf = _FORMAT_ALIASES.get(normal_format)
-        if sf not in ("f32", "f16", "u8") or (nf not in (None, "f32x3", "sn8x3")):
+        if sf not in ("f32", "f16", "u8") or (nf not in (None, "f32x3", "sn8x3", "f16x3")):
             raise ValueError(f"unsupported formats scalar={scalar_format!r} normal={normal_format!r}")
+        _lib.check_formats(sf, nf or "none")
         if device_normals and nf is None:
             raise ValueError("device_normals needs a normal_format")
         dv = _DeviceVolume(self.L, self.ctx, volume_data.shape, sf, nf or "none")
@@ -236,7 +237,7 @@ class HipManager:
         if len(normal_data.shape) != 4:
             raise ValueError("Normal volume must have shape (s0, s1, s2, 3)")
         nf = _FORMAT_ALIASES.get(normal_format)
-        if nf not in ("f32x3", "sn8x3"):
+        if nf not in ("f32x3", "sn8x3", "f16x3"):
             raise ValueError(f"unsupported normal format {normal_format!r}")
         shape = tuple(normal_data.shape[:3])
         data = np.asarray(normal_data, dtype=np.float32)

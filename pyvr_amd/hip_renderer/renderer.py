@@ -9,7 +9,8 @@ reference-identical defaults:
 
 * ``device`` -- HIP device index;
 * ``scalar_format`` / ``normal_format`` -- device storage of the volume
-  ("f32"/"f16"/"u8" and "f32x3"/"sn8x3"; default float32 like the reference);
+  ("f32"/"f16"/"u8" and "f32x3"/"sn8x3", or "f16x3" with "f16"; default
+  float32 like the reference);
 * ``set_opacity_threshold`` -- early-termination alpha (default 0.99, the
   reference shader's constant);
 * ``sample_counts`` / ``last_kernel_ms`` -- measurement helpers.

@@ -33,7 +33,7 @@ from .batch import BatchRenderer, DeviceBlobVolume
 
 _TEXEL_BYTES = {("f32", "none"): 4, ("f16", "none"): 2, ("u8", "none"): 1,
                 ("f32", "f32x3"): 16, ("f16", "f32x3"): 16, ("u8", "f32x3"): 16,
-                ("f32", "sn8x3"): 8, ("f16", "sn8x3"): 8, ("u8", "sn8x3"): 4}
+                ("f32", "sn8x3"): 8, ("f16", "sn8x3"): 8, ("u8", "sn8x3"): 4, ("f16", "f16x3"): 8}
 
 
 def volume_device_bytes(shape, scalar_format: str, normal_format: str, has_normals: bool,

@@ -281,4 +281,6 @@ def dequant_normals(n: np.ndarray, fmt: str) -> np.ndarray:
     if fmt == "sn8x3":
         q = np.rint(np.clip(a, -1.0, 1.0) * np.float32(127.0)).astype(np.float32)
         return q  # direction only matters: the renderer normalises
+    if fmt == "f16x3":
+        return a.astype(np.float16).astype(np.float32)
     raise ValueError(fmt)

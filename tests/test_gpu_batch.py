@@ -107,7 +107,8 @@ def test_batch_replays_and_reuse(hip):
 
 
 @pytest.mark.parametrize("sf,nf,lb,normals", [("f32", "f32x3", 4, True), ("u8", "sn8x3", 3, True),
-                                               ("f16", "sn8x3", 3, False), ("f32", "f32x3", 3, False)])
+                                               ("f16", "sn8x3", 3, False), ("f32", "f32x3", 3, False),
+                                               ("f16", "f16x3", 3, True), ("f16", "f16x3", 4, True)])
 def test_batch_formats_bricks_and_no_normals(hip, sf, nf, lb, normals):
     vols = scenes.c3_volumes(scale=4, with_normals=normals)
     reqs = scenes.c3_requests(6, seed=11, image_scale=4)

@@ -2,7 +2,7 @@
 
 * Normals computed on the device (hvr_volume_upload_scalar_normals) are the
   bytes compute_normal_volume gives on the host: renders with either are
-  identical, image and sample counts, for FP32x3 and SNORM8x3 storage and
+  identical, image and sample counts, for FP32x3, SNORM8x3 and FP16x3 storage and
   non-cubic shapes.
 * Blob volumes generated on the device (hvr_volume_generate_blobs) match a
   NumPy restatement of the generator (same axes, product order and hash
@@ -34,7 +34,7 @@ def render(volume, sc, sf, nf, **load_kw):
 
 
 @pytest.mark.parametrize("shape", [(48, 48, 48), (20, 33, 17)])
-@pytest.mark.parametrize("sf,nf", [("f32", "f32x3"), ("f16", "sn8x3")])
+@pytest.mark.parametrize("sf,nf", [("f32", "f32x3"), ("f16", "sn8x3"), ("f16", "f16x3")])
 def test_device_normals_equal_host_normals(hip, shape, sf, nf):
     from pyvr_amd.datasets import blob_field, compute_normal_volume
     from pyvr_amd.volume import Volume

@@ -1,5 +1,6 @@
 """Storage formats on the GPU: scalar FP32 / FP16 / UNORM8 x normals FP32x3
-/ SNORM8x3, the gather done in the stored format, compositing in FP32.
+/ SNORM8x3, and FP16 scalars with FP16x3 normals; the gather done in the
+stored format, compositing in FP32.
 
 * Parity per format: the kernel against the FP64 oracle run on the
   dequantised data (what the format stores), bar <= 1 LSB per channel on
@@ -30,9 +31,9 @@ from parity import assert_parity, compare, oracle_render, render_hip
 
 pytestmark = pytest.mark.gpu
 
-FORMATS = [(s, n) for s in ("f32", "f16", "u8") for n in ("f32x3", "sn8x3")]
+FORMATS = [(s, n) for s in ("f32", "f16", "u8") for n in ("f32x3", "sn8x3")] + [("f16", "f16x3")]
 SCALAR_BYTES = {"f32": 4, "f16": 2, "u8": 1}
-NORMAL_BYTES = {"f32x3": 12, "sn8x3": 3}
+NORMAL_BYTES = {"f32x3": 12, "sn8x3": 3, "f16x3": 6}
 
 
 @pytest.mark.parametrize("sf,nf", FORMATS)
@@ -131,5 +132,5 @@ def test_precision_sweep_512(hip):
     assert by[("f32", "f32x3")]["vs_fp32_oracle"]["frac_le1"] >= 0.999
     assert by[("f32", "f32x3")]["frac_le1"] >= floor["frac_le1"] - 0.002
     # FP16 scalars add next to nothing to the FP32 floor
-    for nf in ("f32x3", "sn8x3"):
+    for nf in ("f32x3", "sn8x3", "f16x3"):
         assert by[("f16", nf)]["frac_le1"] >= floor["frac_le1"] - 0.005

@@ -244,6 +244,21 @@ class TestSamplerSemantics:
         hs = list(fake.volumes.values())
         assert hs[-1]["sf"] == _lib.HVR_SCALAR["u8"] and hs[-1]["nf"] == _lib.HVR_NORMAL["sn8x3"]
 
+    def test_f16x3_normals_pair_with_f16_scalars_only(self, fake):
+        r = VolumeRenderer(width=8, height=8, scalar_format="f16", normal_format="f16x3")
+        r.load_volume(vol())
+        h = list(fake.volumes.values())[-1]
+        assert h["sf"] == _lib.HVR_SCALAR["f16"] and h["nf"] == _lib.HVR_NORMAL["f16x3"]
+        n = len(fake.volumes)
+        for sf in ("f32", "u8"):
+            with pytest.raises(ValueError, match="f16x3"):
+                r.load_volume(vol(), scalar_format=sf)
+        assert len(fake.volumes) == n              # rejected before any device call
+        m = HipManager()
+        m.load_shaders()
+        m.create_normal_texture(np.zeros((4, 4, 4, 3), np.float32), normal_format="f16x3")
+        assert list(fake.volumes.values())[-1]["nf"] == _lib.HVR_NORMAL["f16x3"]
+
     def test_normals_with_other_shape_get_their_own_volume(self, fake):
         m = HipManager()
         m.load_shaders()
