@@ -1,23 +1,38 @@
 #!/usr/bin/env python3
 """Benchmark of the HIP volume renderer on one MI355X.
 
-Metric: ray samples/s (plus fps and p50/p99 frame latency) of the plain
-one-request-per-launch renderer.  A *step* renders the workload's camera set
-once (default: 24 cameras of the 512^3 blob volume at 1024x1024, high_quality
-preset, FP16 scalar + SNORM8x3 normals), each frame into its own resident
-device buffer, with the volume already resident in HBM.
+Metric: ray samples/s (plus fps and p50/p99 latency) at 512^3 / 1024^2,
+batched form against the plain one-request-per-launch kernel.  A *step*
+renders the workload's camera set once (default: 24 cameras of the 512^3
+blob volume at 1024x1024, high_quality preset, FP16 scalar + SNORM8x3
+normals), with the volume already resident in HBM.
 
+* plain leg: each frame one launch (ray setup + march) into its own resident
+  device buffer, back to back on one stream; its figures are the
+  ``plain_kernel`` object of the line (and the whole line for c0 / c1, or
+  with --plain-only);
+* batched leg (c2): the 24 cameras as 24 requests of one persistent
+  graph-captured launch per step (BatchRenderer; brick size and ray tile
+  from the autotuner), two steps in flight, every step ending with its 24
+  images in pinned host memory; ``value`` is this leg's;
 * samples: reference-algorithm samples (pixel, i) -- i < max_steps, alpha
-  below the termination threshold, sample inside the box -- counted per
-  camera by the renderer's counting pass before the timed region (the tests
-  pin that count to the oracle's);
+  below the termination threshold, sample inside the box -- counted by the
+  renderer's counting pass before the timed region (the tests pin that count
+  to the oracle's);
 * value = samples per step * steps / wall time of the timed steps;
-* roofline: the march kernel, HBM-bound; algorithmic bytes per launch =
-  samples * 8 * (scalar + normal bytes per voxel, unpadded) + 4 * W * H,
-  divided by the kernel's average duration from HIP events recorded on its
-  stream over the timed region;
-* fps / latency: end-to-end VolumeRenderer.render() (set_camera + draw +
-  copy of the RGBA8 image to the host), per frame;
+* roofline: the dominant kernel (k_batch for the batched leg, k_march for
+  the plain one) against HBM peak; algorithmic bytes per launch = samples *
+  8 * (scalar + normal bytes per voxel, unpadded) + 4 * W * H per image,
+  divided by the launch's average duration from HIP events recorded on its
+  stream (for the batched leg: around the graph, which also holds the
+  staging copy, a counter fill and the brick-flag kernel).  Most texel reads
+  hit L1/L2, so that rate exceeds HBM peak; ``traffic`` and
+  ``hbm_measured_*`` give the HBM bytes rocprofv3 measured for these
+  sources, and ``profile`` the kernel's VALU-busy fraction (the limit);
+* fps / latency: batched frames per second; ``batch_ms_p50/p99``: one step
+  of the batch from launch to images in host memory; ``frame_ms_p50/p99``:
+  end-to-end VolumeRenderer.render() (set_camera + draw + copy of the RGBA8
+  image to the host) per frame;
 * cpu_baseline: the oracle's FP32 port of the reference shader (its loop
   runs every i < max_steps, as the shader does) on all host cores, whole
   frames of the workload's cameras until ~15 s have elapsed.
@@ -79,19 +94,41 @@ def kernel_source_hash() -> str:
 
 
 def measured_traffic(key: str):
-    """HBM bytes per march launch from the rocprofv3 FETCH_SIZE/WRITE_SIZE
-    passes recorded in profiles/traffic.json (scaled by the copy-kernel
-    calibration), or None when no measurement matches this workload and
-    these kernel sources."""
+    """HBM bytes per launch from the rocprofv3 FETCH_SIZE/WRITE_SIZE passes
+    recorded in profiles/traffic.json (scaled by the copy-kernel
+    calibration), with the profile's other figures of that kernel (VALU
+    busy, resident waves, L2 hit rate), or (None, None, {}) when no
+    measurement matches this workload and these kernel sources."""
     try:
         with open(TRAFFIC_FILE) as f:
             table = json.load(f)
     except (OSError, ValueError):
-        return None, None
+        return None, None, {}
     e = table.get(key)
     if not e or e.get("kernel_sources") != kernel_source_hash():
-        return None, None
-    return e["read_bytes"] + e["write_bytes"], e.get("source")
+        return None, None, {}
+    extra = {k: e[k] for k in ("valu_busy", "resident_waves_frac", "l2_hit") if k in e}
+    return e["read_bytes"] + e["write_bytes"], e.get("source"), extra
+
+
+def roofline_of(kernel, alg_bytes, ms, bpv, key):
+    """The roofline object: algorithmic bytes (8 texel reads per sample at
+    the stored format, plus the images) over the launch time against HBM
+    peak, and, where a profile of these sources exists, the HBM bytes it
+    measured and the kernel's issue figures.  Texels are re-read from L1/L2
+    (neighbouring samples share taps), so the algorithmic rate can exceed
+    HBM peak; the measured rate shows what HBM actually carried."""
+    achieved = alg_bytes / (ms * 1e-3)
+    traffic, src, prof = measured_traffic(key)
+    out = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_source": src, "traffic_key": key,
+           "alg_bytes_per_launch": alg_bytes, "bytes_per_sample": bpv, "kernel": kernel, "avg_kernel_ms": ms}
+    if traffic is not None:
+        hbm = traffic / (ms * 1e-3)
+        out["hbm_measured_GBps"] = hbm / 1e9
+        out["hbm_measured_frac"] = hbm / HBM_PEAK
+        out["profile"] = prof
+    return out
 
 
 def cpu_baseline(sc, cams, args, sf, nf):
@@ -279,6 +316,78 @@ def bench_batch(args):
     print(json.dumps(line), flush=True)
 
 
+def batched_leg(sc, cams, args, sf, nf, s_step):
+    """The workload's cameras as one batch of requests (BatchRenderer: one
+    persistent graph-captured launch per run, brick size and ray tile from
+    the autotuner), two runs in flight on their own streams; every run ends
+    with all its images in pinned host memory.  Returns the figures for the
+    JSON line."""
+    import scenes
+    from pyvr_amd.hip_renderer import BatchRenderer
+
+    vols = [sc.volume()]
+    reqs = [scenes.Request(0, cam, sc.ctf, sc.otf, sc.width, sc.height, sc.config, sc.light, sc.tau)
+            for cam in cams]
+    tuning = None
+    if args.tile_shape == "auto":
+        from pyvr_amd.hip_renderer.autotune import tune
+
+        t0 = time.perf_counter()
+        bricks = (args.brick_log2,) if args.brick_log2 >= 0 else (3, 4)
+        tuning = tune(vols, reqs, sf, nf, bricks=bricks)
+        lb, tile = tuning["brick_log2"], tuning["tile_shape"]
+        log(f"autotuner ({time.perf_counter() - t0:.1f}s): " +
+            ", ".join(f"lb{a}/{t} {ms:.2f} ms" for a, t, ms in tuning["table"]) + f" -> lb{lb}/{tile}")
+    else:
+        lb, tile = (args.brick_log2 if args.brick_log2 >= 0 else 3), args.tile_shape
+    b = BatchRenderer(scalar_format=sf, normal_format=nf, brick_log2=lb, tile_shape=tile, pipeline=2)
+    try:
+        for q in reqs:
+            b.add(vols[0], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
+        b.run(count=True)
+        samples = sum(int(b.counts(i).sum()) for i in range(len(reqs)))
+        if samples != s_step:
+            log(f"warning: batched sample count {samples} differs from the single-frame count {s_step}")
+        # one run at a time: latency of a whole batch (launch to images in host memory)
+        for _ in range(args.warmup):
+            b.run()
+        lat, dev_ms = [], []
+        for _ in range(args.steps):
+            t = time.perf_counter()
+            b.run()
+            lat.append((time.perf_counter() - t) * 1e3)
+            dev_ms.append(b.last_ms())
+        # two runs in flight: the timed region of the value
+        s0 = b.submit()
+        for q in reqs:
+            b.add(vols[0], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
+        s1 = b.submit()
+        b.wait(s0)
+        b.wait(s1)
+        for k in range(args.warmup):
+            b.rerun((s0, s1)[k % 2])
+        b.wait(s0)
+        b.wait(s1)
+        t_start = time.perf_counter()
+        for k in range(args.steps):
+            b.rerun((s0, s1)[k % 2])
+        b.wait(s0)
+        b.wait(s1)
+        wall = time.perf_counter() - t_start
+    finally:
+        b.close()
+    value = samples * args.steps / wall
+    dev = float(np.mean(dev_ms))
+    log(f"batched: {wall * 1e3 / args.steps:.2f} ms/step two in flight, {value / 1e9:.2f} G samples/s; "
+        f"one at a time {np.mean(lat):.2f} ms/step (graph on the device {dev:.2f} ms)")
+    return {"value": value, "ms_per_step": wall * 1e3 / args.steps, "samples": samples,
+            "frames_per_s": len(reqs) * args.steps / wall, "device_ms_per_run": dev,
+            "batch_ms_p50": float(np.percentile(lat, 50)), "batch_ms_p99": float(np.percentile(lat, 99)),
+            "brick_log2": lb, "tile_shape": tile,
+            "autotune": None if tuning is None else [{"brick_log2": a, "tile_shape": t, "device_ms": ms}
+                                                     for a, t, ms in tuning["table"]]}
+
+
 def bench_serve(args):
     """c4: a serving loop over 16 resident FP16+SNORM8x3 blob volumes of
     256^3 to 1024^3 (generated on the device), requests with Zipf(1.1)
@@ -413,6 +522,8 @@ def main():
     ap.add_argument("--latency-frames", type=int, default=48)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="c2: time only the one-frame-per-launch renderer (no batched leg)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="before the timed region, copy this many MiB device-to-device once "
                          "(a known-size kernel for calibrating traffic counters)")
@@ -508,16 +619,9 @@ def main():
 
     bpv = 8 * (SCALAR_BYTES[sf] + (NORMAL_BYTES[nf] if sc.normals is not None else 0))
     alg_bytes = s_step * args.steps * bpv + 4 * npix * nfr.value
-    achieved = alg_bytes / (march_ms.value * 1e-3)
     tkey = (f"{sc.name}/{len(cams)}cams/{sf}+{nf if sc.normals is not None else 'none'}/lb{args.brick_log2}"
             f"/skip{int(used.value)}")
-    traffic, tsrc = measured_traffic(tkey)
-    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK, "traffic": traffic,
-                "traffic_source": tsrc, "traffic_key": tkey,
-                "alg_bytes_per_launch": alg_bytes / nfr.value,
-                "bytes_per_sample": bpv, "kernel": "hvr::k_march",
-                "avg_kernel_ms": march_ms.value / nfr.value}
+    roofline = roofline_of("hvr::k_march", alg_bytes / nfr.value, march_ms.value / nfr.value, bpv, tkey)
 
     # end-to-end frames through the public API (set_camera + render + D2H)
     lat = []
@@ -533,9 +637,37 @@ def main():
     for b in bufs:
         L.hvr_device_free(ctx, b)
 
+    plain = {"value": value, "ms_per_step": ms_step, "march_kernel_ms": march_ms.value / nfr.value,
+             "fps": 1e3 / float(lat.mean()), "frame_ms_p50": float(np.percentile(lat, 50)),
+             "frame_ms_p99": float(np.percentile(lat, 99)),
+             "frames_per_s_device": len(cams) * args.steps / wall, "roofline": roofline}
+    r.close()
+
+    batched = None
+    if args.workload == "c2" and not args.plain_only:
+        batched = batched_leg(sc, cams, args, sf, nf, s_step)
+        alg_b = batched["samples"] * bpv + 4 * npix * len(cams)
+        bkey = f"{sc.name}/{len(cams)}cams/{sf}+{nf}/batch/lb{batched['brick_log2']}/{batched['tile_shape']}"
+        roofline = roofline_of("hvr::k_batch (graph launch: staging copy, counter fill, brick flags, k_batch)",
+                               alg_b, batched["device_ms_per_run"], bpv, bkey)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sc, cams, args, sf, nf)
+
+    if batched is not None:
+        extra = {
+            "fps": batched["frames_per_s"],
+            "batch_ms_p50": batched["batch_ms_p50"], "batch_ms_p99": batched["batch_ms_p99"],
+            "frame_ms_p50": plain["frame_ms_p50"], "frame_ms_p99": plain["frame_ms_p99"],
+            "batched": {k: batched[k] for k in ("device_ms_per_run", "brick_log2", "tile_shape", "autotune")},
+            "plain_kernel": plain,
+            "batched_vs_plain_kernel": batched["value"] / value,
+        }
+        value, ms_step = batched["value"], batched["ms_per_step"]
+    else:
+        extra = {"fps": plain["fps"], "frame_ms_p50": plain["frame_ms_p50"],
+                 "frame_ms_p99": plain["frame_ms_p99"], "frames_per_s_device": plain["frames_per_s_device"]}
 
     line = {
         "metric": METRIC,
@@ -551,8 +683,10 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"{sc.name}: {len(cams)} cameras, {sc.width}x{sc.height}, "
-                        f"{sc.data.shape[0]}^3 blob volume" if args.workload != "c0" else sc.name,
+            "workload": (f"{sc.name}: {len(cams)} cameras, {sc.width}x{sc.height}, "
+                         f"{sc.data.shape[0]}^3 blob volume" if args.workload != "c0" else sc.name)
+                        + ("; the cameras batched into one persistent graph launch per step"
+                           if batched is not None else "; one launch per frame"),
             "volume": list(sc.data.shape), "image": [sc.width, sc.height],
             "cameras": len(cams), "preset": {"step_size": sc.config.step_size,
                                              "max_steps": sc.config.max_steps},
@@ -560,14 +694,10 @@ def main():
             "normal_format": nf if sc.normals is not None else "none",
             "samples_per_step": s_step, "volume_bytes": vol_bytes, "brick_skip": skip,
         },
-        "fps": 1e3 / float(lat.mean()),
-        "frame_ms_p50": float(np.percentile(lat, 50)),
-        "frame_ms_p99": float(np.percentile(lat, 99)),
-        "frames_per_s_device": len(cams) * args.steps / wall,
+        **extra,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    r.close()
     print(json.dumps(line), flush=True)
 
 
