@@ -1,8 +1,5 @@
-"""Camera parameters and placement (API of the reference's ``pyvr.camera``).
-
-The interactive helpers of the reference (``CameraPath``, ``CameraController``,
-trackball) drive the UI, not the render path, and are not part of this package.
-"""
+"""Camera parameters, placement, keyframed paths and interactive control
+(API of the reference's ``pyvr.camera``)."""
 
 from .camera import (
     Camera,
@@ -11,9 +8,11 @@ from .camera import (
     radians_to_degrees,
     validate_camera_angles,
 )
-from .control import get_camera_pos, get_camera_pos_from_params
+from .control import CameraController, CameraPath, get_camera_pos, get_camera_pos_from_params
 
 __all__ = [
+    "CameraController",
+    "CameraPath",
     "get_camera_pos",
     "get_camera_pos_from_params",
     "Camera",

@@ -72,6 +72,24 @@ class ServeRequest:
         return None if self.done is None else self.done - self.arrival
 
 
+def path_requests(volume: Hashable, cameras, ctf, otf, width: int = 512, height: int = 512, config=None,
+                  light=None, tau: float = 0.99, start: Optional[float] = None,
+                  interval: Optional[float] = None, tag=None) -> List[ServeRequest]:
+    """One request per camera of a sequence -- ``CameraPath.generate_frames``,
+    ``CameraController.animate_to`` or any list of ``Camera`` -- on the
+    registered ``volume``, in order.  With ``start`` and ``interval`` the
+    arrivals are ``start + k * interval`` (a frame stream at 1 / interval
+    frames per second); otherwise ``submit`` stamps them.  ``tag`` becomes
+    ``(tag, k)``."""
+    reqs = []
+    for k, cam in enumerate(cameras):
+        arrival = None if start is None or interval is None else start + k * interval
+        reqs.append(ServeRequest(volume=volume, camera=cam, ctf=ctf, otf=otf, width=width, height=height,
+                                 config=config, light=light, tau=tau, arrival=arrival,
+                                 tag=None if tag is None else (tag, k)))
+    return reqs
+
+
 @dataclass
 class _Resident:
     volume: object

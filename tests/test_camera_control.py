@@ -1,0 +1,229 @@
+"""CameraPath and CameraController (pyvr_amd/camera/control.py): the
+reference's expectations (tests/test_camera/test_control.py:114-392,
+tests/test_camera/test_trackball.py), and what the geometry fixes: the
+orientation decomposition inverts get_camera_pos, and a trackball drag from
+the front view turns the camera by the arcball angle asin(d / min(w, h))
+about the expected axis only."""
+
+import numpy as np
+import pytest
+
+from pyvr_amd.camera import Camera, CameraController, CameraPath, get_camera_pos, get_camera_pos_from_params
+from pyvr_amd.camera.control import (
+    _camera_to_quaternion,
+    _map_to_sphere,
+    _quaternion_to_camera_angles,
+)
+
+
+def wrap(a):
+    return (a + np.pi) % (2 * np.pi) - np.pi
+
+
+# -- CameraPath --------------------------------------------------------------
+
+def test_path_front_to_side():
+    a, b = Camera.front_view(distance=3.0), Camera.side_view(distance=3.0)
+    path = CameraPath([a, b])
+    mid = path.interpolate(0.5)
+    assert mid.distance == pytest.approx(3.0) and mid.azimuth == pytest.approx(np.pi / 4)
+    frames = path.generate_frames(5)
+    assert len(frames) == 5
+    assert [f.azimuth for f in frames] == pytest.approx([0, np.pi / 8, np.pi / 4, 3 * np.pi / 8, np.pi / 2])
+    assert frames[0] is not a and np.allclose(frames[-1].target, b.target)
+
+
+def test_path_segments_and_keyframes():
+    ks = [Camera(azimuth=0.0, distance=2.0), Camera(azimuth=1.0, distance=4.0, fov=0.5),
+          Camera(azimuth=2.0, distance=3.0, near_plane=0.2)]
+    path = CameraPath(ks)
+    m = path.interpolate(0.5)             # exactly the middle keyframe (segment 1, u = 0)
+    assert (m.azimuth, m.distance, m.fov) == pytest.approx((1.0, 4.0, 0.5))
+    q = path.interpolate(0.25)            # middle of segment 0
+    assert (q.azimuth, q.distance, q.fov) == pytest.approx((0.5, 3.0, (np.pi / 4 + 0.5) / 2))
+    q = path.interpolate(0.75)            # middle of segment 1
+    assert (q.azimuth, q.distance, q.near_plane) == pytest.approx((1.5, 3.5, 0.15))
+    assert path.generate_frames(1)[0].azimuth == 0.0
+
+
+@pytest.mark.parametrize("a1,a2,mid", [(-0.9 * np.pi, 0.9 * np.pi, -np.pi), (0.9 * np.pi, -0.9 * np.pi, np.pi),
+                                       (0.1, 0.5, 0.3), (3 * np.pi, 0.0, -np.pi / 2)])
+def test_path_shorter_arc(a1, a2, mid):
+    """Angles wrap to [-pi, pi) first (3 pi -> -pi) and take the shorter arc."""
+    m = CameraPath([Camera(azimuth=a1), Camera(azimuth=a2)]).interpolate(0.5)
+    assert abs(wrap(m.azimuth - mid)) < 1e-9
+
+
+def test_path_validation():
+    with pytest.raises(ValueError, match="At least 2 keyframes are required"):
+        CameraPath([Camera.front_view()])
+    with pytest.raises(ValueError, match="must be a Camera instance"):
+        CameraPath(["not a Camera", "not a Camera"])
+    path = CameraPath([Camera.front_view(), Camera.side_view()])
+    for t in (-0.1, 1.1):
+        with pytest.raises(ValueError, match="t must be between 0.0 and 1.0"):
+            path.interpolate(t)
+    with pytest.raises(ValueError, match="n_frames must be at least 1"):
+        path.generate_frames(0)
+
+
+# -- CameraController ----------------------------------------------------------
+
+def test_controller_basics():
+    c = CameraController()
+    assert (c.params.azimuth, c.params.elevation, c.params.distance) == (0.0, 0.0, 3.0)
+    c.orbit(np.pi / 4, np.pi / 6)
+    assert (c.params.azimuth, c.params.elevation) == pytest.approx((np.pi / 4, np.pi / 6))
+    c.orbit(0.0, 10.0)
+    assert c.params.elevation == pytest.approx(np.pi / 2)           # clamped
+    c.zoom(0.5)
+    assert c.params.distance == pytest.approx(1.5)
+    c.zoom(1e-6)
+    assert c.params.distance == pytest.approx(0.01)                 # clamped
+    t0 = c.params.target.copy()
+    c.pan(np.array([1.0, 2.0, 3.0]))
+    assert np.allclose(c.params.target, t0 + [1, 2, 3])
+    c.roll_camera(0.25)
+    assert c.params.roll == pytest.approx(0.25)
+    c.set_distance(4.0)
+    c.look_at(np.array([1, 1, 1]))
+    assert c.params.distance == 4.0 and c.params.target.dtype == np.float32
+    pos, up = c.get_position_and_up()
+    assert np.linalg.norm(pos - c.params.target) == pytest.approx(4.0, rel=1e-6)
+
+
+def test_controller_validation_and_presets():
+    c = CameraController()
+    for f in (-0.5, 0.0):
+        with pytest.raises(ValueError, match="Zoom factor must be positive"):
+            c.zoom(f)
+    with pytest.raises(ValueError, match="delta_target must be a 3D numpy array"):
+        c.pan([1, 2, 3])
+    with pytest.raises(ValueError, match="Distance must be positive"):
+        c.set_distance(-1.0)
+    with pytest.raises(ValueError, match="target must be a 3D numpy array"):
+        c.look_at([0, 0, 0])
+    c.pan(np.array([0.5, 0.0, 0.0]))
+    c.reset_to_preset("side")
+    assert c.params.azimuth == pytest.approx(np.pi / 2) and np.allclose(c.params.target, [0.5, 0, 0])
+    c.reset_to_preset("top", distance=7.0)
+    assert c.params.elevation == pytest.approx(np.pi / 2) and c.params.distance == 7.0
+    c.reset_to_preset("isometric")
+    assert (c.params.azimuth, c.params.elevation) == pytest.approx((np.pi / 4, np.pi / 6))
+    with pytest.raises(ValueError, match="Unknown preset"):
+        c.reset_to_preset("invalid")
+
+
+def test_controller_animation():
+    c = CameraController()
+    goal = Camera.isometric_view(distance=5.0)
+    frames = c.animate_to(goal, n_frames=10)
+    assert len(frames) == 10
+    assert frames[0].azimuth == pytest.approx(c.params.azimuth) and frames[0].distance == pytest.approx(3.0)
+    assert frames[-1].azimuth == pytest.approx(goal.azimuth) and frames[-1].distance == pytest.approx(5.0)
+
+
+# -- trackball -------------------------------------------------------------------
+
+def test_map_to_sphere():
+    assert np.allclose(_map_to_sphere(0.0, 0.0), [0, 0, 1])
+    for x, y in [(0.3, 0.3), (0.5, 0.5), (1.5, 1.5), (0.5, 0.3)]:
+        p = _map_to_sphere(x, y)
+        assert abs(np.linalg.norm(p) - 1.0) < 1e-6 and p.dtype == np.float32
+    # inside r/sqrt(2): the sphere itself (z^2 = 1 - x^2 - y^2, already unit)
+    assert np.allclose(_map_to_sphere(0.3, 0.4), [0.3, 0.4, np.sqrt(0.75)], atol=1e-7)
+    # outside: the hyperbolic sheet z = 0.5 / |p|, then normalised
+    p = np.array([1.5, 1.5, 0.5 / np.hypot(1.5, 1.5)])
+    assert np.allclose(_map_to_sphere(1.5, 1.5), p / np.linalg.norm(p), atol=1e-7)
+    assert _map_to_sphere(0.5, 0.5, radius=2.0)[2] > _map_to_sphere(0.5, 0.5)[2]
+    a, b = _map_to_sphere(0.5, 0.3), _map_to_sphere(0.3, 0.5)
+    assert np.allclose([a[0], a[1], a[2]], [b[1], b[0], b[2]])
+
+
+def test_orientation_decomposition_inverts_placement():
+    """For any camera whose initial offset is perpendicular to init_up (the
+    convention every preset uses), decomposing its orientation returns its
+    own angles (modulo 2 pi), for elevations inside (-pi/2, pi/2)."""
+    rng = np.random.default_rng(7)
+    for _ in range(200):
+        t = rng.uniform(-1, 1, 3).astype(np.float32)
+        phi = rng.uniform(0, 2 * np.pi)
+        ip = (t + np.array([np.cos(phi), np.sin(phi), 0.0]) * rng.uniform(0.5, 2)).astype(np.float32)
+        cam = Camera(target=t, azimuth=rng.uniform(-np.pi, np.pi), elevation=rng.uniform(-1.5, 1.5),
+                     roll=rng.uniform(-np.pi, np.pi), distance=rng.uniform(0.5, 5), init_pos=ip)
+        q = _camera_to_quaternion(cam)
+        assert abs(np.linalg.norm(q.as_quat()) - 1) < 1e-12
+        az, el, ro = _quaternion_to_camera_angles(q, cam.target, cam.distance, cam.init_pos, cam.init_up)
+        assert max(abs(wrap(az - cam.azimuth)), abs(el - cam.elevation), abs(wrap(ro - cam.roll))) < 5e-6
+    # at a pole the azimuth is conventionally 0 and the elevation still exact
+    for el in (np.pi / 2, -np.pi / 2):
+        cam = Camera(azimuth=np.pi / 4, elevation=el)
+        az, e, _ = _quaternion_to_camera_angles(_camera_to_quaternion(cam), cam.target, cam.distance,
+                                                cam.init_pos, cam.init_up)
+        assert e == pytest.approx(el, abs=1e-5)
+
+
+@pytest.mark.parametrize("w,h", [(800, 600), (600, 1600)])
+def test_trackball_closed_form_from_front_view(w, h):
+    """A horizontal drag of d pixels moves the arcball point to x = -d/s
+    (s = min(w, h)); the rotation about the camera's up axis by asin(d/s)
+    lowers the azimuth by exactly that, nothing else; a vertical drag of -d
+    raises the elevation by asin(d/s)."""
+    s = min(w, h)
+    for d in (50, 100, -120):
+        c = CameraController(Camera.front_view(distance=3.0))
+        c.trackball(dx=d, dy=0, viewport_width=w, viewport_height=h)
+        assert c.params.azimuth == pytest.approx(-np.arcsin(d / s), abs=1e-6)
+        assert abs(c.params.elevation) < 1e-9 and abs(c.params.roll) < 1e-9
+        c = CameraController(Camera.front_view(distance=3.0))
+        c.trackball(dx=0, dy=-d, viewport_width=w, viewport_height=h)
+        assert c.params.elevation == pytest.approx(np.arcsin(d / s), abs=1e-6)
+        assert abs(c.params.azimuth) < 1e-9 and abs(c.params.roll) < 1e-9
+    # sensitivity scales the arcball point
+    c = CameraController(Camera.front_view(distance=3.0))
+    c.trackball(dx=50, dy=0, viewport_width=w, viewport_height=h, sensitivity=2.0)
+    assert c.params.azimuth == pytest.approx(-np.arcsin(100 / s), abs=1e-6)
+
+
+def test_trackball_behaviour():
+    c = CameraController(Camera.front_view(distance=3.0))
+    c.trackball(dx=100, dy=-100, viewport_width=800, viewport_height=600)
+    assert c.params.azimuth < 0 and c.params.elevation > 0
+    iso = Camera.isometric_view(target=np.array([1.0, 2.0, 3.0], np.float32), distance=5.0)
+    c = CameraController(iso)
+    before = (c.params.azimuth, c.params.elevation, c.params.roll)
+    c.trackball(dx=0, dy=0, viewport_width=800, viewport_height=600)
+    c.trackball(dx=0.1, dy=0.1, viewport_width=800, viewport_height=600)   # below 0.001 of the viewport
+    assert (c.params.azimuth, c.params.elevation, c.params.roll) == before
+    pos0, _ = get_camera_pos_from_params(c.params)
+    c.trackball(dx=50, dy=30, viewport_width=800, viewport_height=600)
+    c.trackball(dx=-30, dy=50, viewport_width=800, viewport_height=600)
+    pos1, _ = get_camera_pos_from_params(c.params)
+    assert c.params.distance == 5.0 and np.allclose(c.params.target, [1, 2, 3])
+    assert np.linalg.norm(pos1 - c.params.target) == pytest.approx(5.0, rel=1e-6)
+    assert not np.allclose(pos0, pos1)
+    for bad in ((0, 600), (800, -100)):
+        with pytest.raises(ValueError, match="Viewport dimensions must be positive"):
+            c.trackball(dx=10, dy=10, viewport_width=bad[0], viewport_height=bad[1])
+    c = CameraController(Camera.front_view(distance=3.0))
+    for _ in range(5):
+        c.trackball(dx=20, dy=0, viewport_width=800, viewport_height=600)
+    assert c.params.azimuth == pytest.approx(-5 * np.arcsin(20 / 600), abs=1e-5)   # rotations about one axis add
+
+
+def test_trackball_composes_in_camera_frame():
+    """From any orientation, a drag turns the camera about an axis of its own
+    frame: the new back vector is the old frame times the arcball rotation."""
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        cam = Camera(azimuth=rng.uniform(-3, 3), elevation=rng.uniform(-1.2, 1.2), roll=rng.uniform(-3, 3))
+        m0 = _camera_to_quaternion(cam).as_matrix()
+        c = CameraController(cam)
+        dx, dy = rng.uniform(-200, 200, 2)
+        c.trackball(dx=dx, dy=dy, viewport_width=800, viewport_height=600)
+        x, y = -dx / 600, -dy / 600
+        p1 = _map_to_sphere(x, y).astype(np.float64)
+        expected_back = m0 @ p1          # the rotation maps local +z to the arcball point
+        pos, _ = get_camera_pos_from_params(c.params)
+        back = (pos - c.params.target) / c.params.distance
+        assert np.allclose(back, expected_back, atol=1e-5)

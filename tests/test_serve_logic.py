@@ -196,3 +196,27 @@ def test_rejects_what_can_never_fit():
         srv.register("big", vol("big", 64))
     with pytest.raises(KeyError):
         srv.submit(req("nope"))
+
+
+def test_camera_path_requests_are_served_in_order():
+    """A CameraPath turned into a request stream: one request per frame, in
+    order, with the path's cameras and the given arrival spacing; the server
+    batches them in that order."""
+    from pyvr_amd.camera import Camera, CameraController, CameraPath
+    from pyvr_amd.hip_renderer.serve import path_requests
+
+    frames = CameraPath([Camera.front_view(), Camera.side_view(), Camera.top_view()]).generate_frames(7)
+    reqs = path_requests("A", frames, None, None, width=16, height=16, start=10.0, interval=0.02, tag="orbit")
+    assert [r.camera for r in reqs] == frames and [r.tag for r in reqs] == [("orbit", k) for k in range(7)]
+    assert [r.arrival for r in reqs] == pytest.approx([10.0 + 0.02 * k for k in range(7)])
+    srv, be = make(3, max_batch_requests=4)
+    for r in reqs:
+        srv.submit(r)
+    srv.poll()
+    assert [len(l) for l in be.launches] == [4, 3]
+    be.finish(0)
+    be.finish(1)
+    done = srv.drain()
+    assert [r.tag[1] for r in done] == list(range(7))
+    moves = CameraController().animate_to(Camera.isometric_view(), n_frames=5)
+    assert all(r.arrival is None for r in path_requests("B", moves, None, None))

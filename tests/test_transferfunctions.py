@@ -1,6 +1,7 @@
 """Transfer functions: known answers of the reference's tests
 (tests/test_transferfunctions/test_color.py:17-176, test_opacity.py:16-100)
-and the listed-colormap fixtures in tests/golden/colormaps.npz."""
+and the colormap fixtures in tests/golden/colormaps.npz (the 12 maps of the
+reference's colour selector)."""
 
 import os
 
@@ -49,13 +50,29 @@ def test_color_queries():
     assert np.allclose(g(64), g.to_lut(64))
 
 
-@pytest.mark.parametrize("name", ["viridis", "plasma", "inferno", "magma"])
+UI_COLORMAPS = ["viridis", "plasma", "inferno", "magma", "cividis", "gray", "bone", "copper", "hot", "cool",
+                "turbo", "jet"]
+
+
+@pytest.mark.parametrize("name", UI_COLORMAPS)
 def test_from_colormap_reproduces_listed_colours(name):
     ref = np.load(GOLDEN)[name]
     ctf = ColorTransferFunction.from_colormap(name)
     assert len(ctf.control_points) == 256
     lut = ctf.to_lut(256)
     assert np.array_equal(lut, ref.astype(np.float32))
+
+
+def test_segmented_colormaps_closed_forms():
+    """gray is (x, x, x) and cool is (x, 1 - x, 1) at x = i / 255: the
+    fixtures and from_colormap agree with the formulas, not only with the
+    installed matplotlib."""
+    x = np.linspace(0.0, 1.0, 256)
+    g = np.load(GOLDEN)
+    assert np.allclose(g["gray"], np.stack([x, x, x], 1), atol=1e-12)
+    assert np.allclose(g["cool"], np.stack([x, 1 - x, np.ones_like(x)], 1), atol=1e-12)
+    lut = ColorTransferFunction.from_colormap("cool").to_lut(256)
+    assert np.array_equal(lut, np.stack([x, 1 - x, np.ones_like(x)], 1).astype(np.float32))
 
 
 def test_from_colormap_range_and_errors():
