@@ -180,6 +180,13 @@ int hvr_render_async(hvr_ctx *ctx);
 int hvr_read(hvr_ctx *ctx, uint8_t *out);
 int hvr_synchronize(hvr_ctx *ctx);
 
+/* The read step without a host copy (replaces fbo.read of read_pixels,
+ * manager.py:228-230): copies the framebuffer into the context's pinned host
+ * buffer, waits, and sets *rgba to it (width*height*4 bytes, bottom row
+ * first; valid until the next read or render on this context), so that a
+ * caller building its own object (a Python bytes) copies the image once. */
+int hvr_read_pinned(hvr_ctx *ctx, const uint8_t **rgba);
+
 /* Device pointer of the context's RGBA8 framebuffer (width*height*4 bytes). */
 int hvr_framebuffer(hvr_ctx *ctx, void **device_ptr);
 

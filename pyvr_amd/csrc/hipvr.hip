@@ -978,6 +978,18 @@ int hvr_read(hvr_ctx *c, uint8_t *out)
     return HVR_OK;
 }
 
+int hvr_read_pinned(hvr_ctx *c, const uint8_t **rgba)
+{
+    if (!c || !rgba) return fail(HVR_EINVAL, "null argument");
+    *rgba = nullptr;
+    HIPCHK(hipSetDevice(c->device));
+    const size_t bytes = (size_t)c->W * c->H * 4;
+    HIPCHK(hipMemcpyAsync(c->h_pinned, c->d_out, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *rgba = (const uint8_t *)c->h_pinned;
+    return HVR_OK;
+}
+
 int hvr_render(hvr_ctx *c, uint8_t *out)
 {
     int rc = hvr_render_async(c);

@@ -41,7 +41,7 @@ EXPORTS = (
     "hvr_skip_info", "hvr_batch_create", "hvr_batch_destroy", "hvr_batch_clear", "hvr_batch_set_tile_shape",
     "hvr_batch_add", "hvr_batch_run_async", "hvr_batch_wait", "hvr_batch_run", "hvr_batch_output",
     "hvr_batch_counts", "hvr_batch_last_ms", "hvr_volume_upload_scalar_normals", "hvr_volume_generate_blobs",
-    "hvr_batch_query", "hvr_device_memory",
+    "hvr_batch_query", "hvr_device_memory", "hvr_read_pinned",
 )
 
 # batch ray-tile shapes (hvr_batch_set_tile_shape)
@@ -96,6 +96,7 @@ def _declare(L):
         "hvr_render": ([vp, u8p], i),
         "hvr_render_async": ([vp], i),
         "hvr_read": ([vp, u8p], i),
+        "hvr_read_pinned": ([vp, ctypes.POINTER(u8p)], i),
         "hvr_synchronize": ([vp], i),
         "hvr_framebuffer": ([vp, ctypes.POINTER(vp)], i),
         "hvr_sample_counts": ([vp, ctypes.POINTER(ctypes.c_uint32)], i),

@@ -404,10 +404,11 @@ class HipManager:
         check(L.hvr_render_async(self.ctx), "hvr_render_async")
 
     def read_pixels(self) -> bytes:
-        """Framebuffer contents: width*height*4 bytes, RGBA8, bottom row first."""
-        buf = (ctypes.c_uint8 * (self.width * self.height * 4))()
-        check(self.L.hvr_read(self.ctx, buf), "hvr_read")
-        return bytes(buf)
+        """Framebuffer contents: width*height*4 bytes, RGBA8, bottom row first
+        (copied once, from the context's pinned host buffer)."""
+        ptr = ctypes.POINTER(ctypes.c_uint8)()
+        check(self.L.hvr_read_pinned(self.ctx, ctypes.byref(ptr)), "hvr_read_pinned")
+        return ctypes.string_at(ptr, self.width * self.height * 4)
 
     def sample_counts(self) -> np.ndarray:
         """Samples composited per pixel for the current state (H, W) uint32,

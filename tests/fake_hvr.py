@@ -100,6 +100,14 @@ class FakeHvr:
         ctypes.memmove(buf, data, len(data))
         return 0
 
+    def hvr_read_pinned(self, ctx, ref):
+        w, h = self.contexts[_hv(ctx)]
+        buf = (ctypes.c_uint8 * (4 * w * h))()
+        self.hvr_read(ctx, buf)
+        self._pinned = buf                       # kept alive like the library's buffer
+        ref._obj.contents = ctypes.c_uint8.from_buffer(buf)
+        return 0
+
     def hvr_sample_counts(self, ctx, ptr):
         return 0
 
