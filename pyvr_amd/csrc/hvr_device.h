@@ -682,8 +682,10 @@ __device__ __forceinline__ float4 lut_global(const float4 *__restrict__ lut, int
 // diffuse light), front-to-back composite.
 __device__ __forceinline__ float lambert(const FrameConst &fc, float n0, float n1, float n2)
 {
+    // v_rsq_f32 without the subnormal-input rescaling of rsqrtf: the result
+    // is only used for nn >= FLT_MIN, where the two agree
     const float nn = fmaf(n0, n0, fmaf(n1, n1, n2 * n2));
-    const float dt = fmaf(n0, fc.L[0], fmaf(n1, fc.L[1], n2 * fc.L[2])) * rsqrtf(nn);
+    const float dt = fmaf(n0, fc.L[0], fmaf(n1, fc.L[1], n2 * fc.L[2])) * __builtin_amdgcn_rsqf(nn);
     return (fc.L_ok && nn >= 1.17549435e-38f) ? fmaxf(dt, 0.f) : 0.f;
 }
 
