@@ -1086,11 +1086,13 @@ static int batch_launch(const char *blk, int cls, uint32_t *ctr, int lut_cap, in
 {
     const size_t lds = 2 * (size_t)lut_cap * sizeof(float4);
     int per = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch<SF, NM, LB, COUNT, TS, BUF>, 256, lds) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch<SF, NM, LB, COUNT, TS, BUF>, 64 * kBatchWaves,
+                                                     lds) !=
             hipSuccess ||
         per < 1)
         per = 1;
-    hipLaunchKernelGGL((k_batch<SF, NM, LB, COUNT, TS, BUF>), dim3(per * ncu), dim3(256), lds, st, blk, cls, ctr);
+    hipLaunchKernelGGL((k_batch<SF, NM, LB, COUNT, TS, BUF>), dim3(per * ncu), dim3(64 * kBatchWaves), lds, st, blk,
+                       cls, ctr);
     return hipGetLastError() == hipSuccess ? HVR_OK : HVR_EDEVICE;
 }
 
@@ -1317,7 +1319,9 @@ int hvr_batch_run_async(hvr_batch *b, int count)
             return (long long)b->reqs[a].W * b->reqs[a].H > (long long)b->reqs[z].W * b->reqs[z].H;
         });
     // sizes
-    const int bx = b->tile_shape == HVR_TILE_16X4 ? 32 : 16, by = b->tile_shape == HVR_TILE_16X4 ? 8 : 16;
+    // the block tile of k_batch: 2 waves across, kBatchWaves / 2 down
+    const int bx = b->tile_shape == HVR_TILE_16X4 ? 32 : 16;
+    const int by = (b->tile_shape == HVR_TILE_16X4 ? 4 : 8) * (kBatchWaves / 2);
     size_t desc_off = align_up(sizeof(BatchHeader), 256);
     size_t lut_off = align_up(desc_off + (size_t)nreq * sizeof(ReqDesc), 256);
     size_t lut_bytes = 0, nz_bytes = 0, pixels = 0, flag_bytes = 0;

@@ -1240,7 +1240,14 @@ struct BatchHeader {
     int pad;
 };
 
-// Ray-tile shapes: a wave covers WX x WY pixels, a block 2 x 2 waves.
+// Ray-tile shapes: a wave covers WX x WY pixels; a block is kBatchWaves
+// waves, 2 across and kBatchWaves / 2 down, and takes one tile of
+// (2 WX) x (kBatchWaves / 2 * WY) pixels at a time.  Measured on 24 frames
+// of the 512^3 volume and on the 64-request mix: 2 waves per block (half
+// the tile, twice the counter traffic) and 8 waves (a longer wait at the
+// per-tile barrier) are both slower than 4.
+static constexpr int kBatchWaves = 4;
+
 template <int TS>
 struct TileShape;
 template <>
@@ -1253,12 +1260,12 @@ struct TileShape<1> {
 };
 
 template <int SF, int NM, int LB, bool COUNT, int TS, bool BUF>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(64 * kBatchWaves)
 k_batch(const char *__restrict__ block, int cls, uint32_t *__restrict__ counter)
 {
     extern __shared__ float4 s_lut[];
     __shared__ int s_item[2];
-    constexpr int WX = TileShape<TS>::WX, WY = TileShape<TS>::WY, BX = 2 * WX, BY = 2 * WY;
+    constexpr int WX = TileShape<TS>::WX, WY = TileShape<TS>::WY, BX = 2 * WX, BY = kBatchWaves / 2 * WY;
     const BatchHeader *hdr = (const BatchHeader *)block;
     const int nreq = hdr->cls[cls].nreq;
     if (nreq <= 0) return;
