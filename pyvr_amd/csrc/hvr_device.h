@@ -238,39 +238,50 @@ struct Footprint {
     uint32_t off;     // its byte offset in the allocation (buffer-load path)
     float u, v, w;    // extended texel coordinates along texture x, y, z (>= 0 in
                       // the box; the weights are their fractional parts)
+    uint32_t X, Y, Z; // the lower-corner texel (floor of u, v, w)
     uint32_t brick;   // brick holding the whole footprint
 };
 
-template <int T, int LB, bool BUF>
-__device__ __forceinline__ Footprint footprint(const TexDesc &t, float w0, float w1, float w2)
+// A footprint in two steps: the coordinates and the brick (all that the
+// transparent-brick test needs), then the texel address (what the loads
+// need), so that samples in transparent bricks skip the second.
+template <int LB, bool BUF>
+__device__ __forceinline__ Footprint footprint_brick(const TexDesc &t, float w0, float w1, float w2)
 {
     // u, v, w >= 0 for in-box positions (and the 0 the callers pass for
     // others), so truncation is floor
-    const float u = fmaf(w0, t.k0, 0.5f), v = fmaf(w1, t.k1, 0.5f), w = fmaf(w2, t.k2, 0.5f);
     Footprint f;
-    f.u = u;
-    f.v = v;
-    f.w = w;
-    const uint32_t X = (uint32_t)u, Y = (uint32_t)v, Z = (uint32_t)w;
-    constexpr uint32_t M = (1u << LB) - 1u, E = (1u << LB) + 1u;
-    const uint32_t local = ((Z & M) * E + (Y & M)) * E + (X & M);
-    uint32_t brick;
+    f.u = fmaf(w0, t.k0, 0.5f);
+    f.v = fmaf(w1, t.k1, 0.5f);
+    f.w = fmaf(w2, t.k2, 0.5f);
+    f.X = (uint32_t)f.u;
+    f.Y = (uint32_t)f.v;
+    f.Z = (uint32_t)f.w;
     if constexpr (BUF) {
         // The buffer path is only used with nbricks < 2^24 (and the
         // allocation below 4 GiB): every partial brick index and
         // brick_bytes (<= 17^3 * 16) fit 24 bits and the byte offset 32, so
         // single-pass 24-bit multiplies are exact.
-        brick = __umul24(__umul24(Z >> LB, t.nby) + (Y >> LB), t.nbx) + (X >> LB);
-        f.off = __umul24(brick, t.brick_bytes) + local * (uint32_t)T;   // < nbytes32 < 2^32
-        f.p = nullptr;
+        f.brick = __umul24(__umul24(f.Z >> LB, t.nby) + (f.Y >> LB), t.nbx) + (f.X >> LB);
     } else {
-        brick = ((Z >> LB) * t.nby + (Y >> LB)) * t.nbx + (X >> LB);
-        f.p = (const char *)t.base + (uint64_t)brick * t.brick_bytes + (uint64_t)(local * (uint32_t)T);
-        f.off = 0;
+        f.brick = ((f.Z >> LB) * t.nby + (f.Y >> LB)) * t.nbx + (f.X >> LB);
     }
-    f.brick = brick;
+    f.p = nullptr;
+    f.off = 0;
     return f;
 }
+
+template <int T, int LB, bool BUF>
+__device__ __forceinline__ void footprint_addr(const TexDesc &t, Footprint &f)
+{
+    constexpr uint32_t M = (1u << LB) - 1u, E = (1u << LB) + 1u;
+    const uint32_t local = ((f.Z & M) * E + (f.Y & M)) * E + (f.X & M);
+    if constexpr (BUF)
+        f.off = __umul24(f.brick, t.brick_bytes) + local * (uint32_t)T;   // < nbytes32 < 2^32
+    else
+        f.p = (const char *)t.base + (uint64_t)f.brick * t.brick_bytes + (uint64_t)(local * (uint32_t)T);
+}
+
 
 // Where the march reads texels from: a buffer resource (base and size in
 // scalar registers, 32-bit per-lane offsets, reads past the allocation
@@ -796,8 +807,11 @@ __device__ __forceinline__ void march_ray(const FrameConst &fc, const TexDesc &t
         }
     } else {
         const Src<BUF> src(ts);
-        // one sample's contribution from its footprint
-        auto shade = [&](const Footprint &f) {
+        // one sample's contribution from its footprint (coordinates and
+        // brick; the address is completed here)
+        auto shade = [&](const Footprint &fb) {
+            Footprint f = fb;
+            footprint_addr<T, LB, BUF>(ts, f);
             if constexpr (NM == NM_SAME) {
                 constexpr int T = texel_bytes(SF);
                 uint32_t raw[4][T / 2];
@@ -835,7 +849,7 @@ __device__ __forceinline__ void march_ray(const FrameConst &fc, const TexDesc &t
                 const float wx = x - fc.bmin[0], wy = y - fc.bmin[1], wz = z - fc.bmin[2];
                 if (!inbox(fc, wx, wy, wz)) break;
                 if constexpr (COUNT) ++ns;
-                shade(footprint<T, LB, BUF>(ts, wz, wy, wx));
+                shade(footprint_brick<LB, BUF>(ts, wz, wy, wx));
                 x = __fadd_rn(x, r.dsx);
                 y = __fadd_rn(y, r.dsy);
                 z = __fadd_rn(z, r.dsz);
@@ -845,7 +859,7 @@ __device__ __forceinline__ void march_ray(const FrameConst &fc, const TexDesc &t
             // in a transparent brick is counted without loading texels.
             float wx = x - fc.bmin[0], wy = y - fc.bmin[1], wz = z - fc.bmin[2];
             bool in = i < nlim && inbox(fc, wx, wy, wz);
-            Footprint f = footprint<T, LB, BUF>(ts, in ? wz : 0.f, in ? wy : 0.f, in ? wx : 0.f);
+            Footprint f = footprint_brick<LB, BUF>(ts, in ? wz : 0.f, in ? wy : 0.f, in ? wx : 0.f);
             bool clear = in && ld_global(skip + f.brick) != 0;
             while (in && A < fc.tau) {
                 if (!clear) shade(f);
@@ -858,7 +872,7 @@ __device__ __forceinline__ void march_ray(const FrameConst &fc, const TexDesc &t
                 wy = y - fc.bmin[1];
                 wz = z - fc.bmin[2];
                 in = i < nlim && inbox(fc, wx, wy, wz);
-                f = footprint<T, LB, BUF>(ts, in ? wz : 0.f, in ? wy : 0.f, in ? wx : 0.f);
+                f = footprint_brick<LB, BUF>(ts, in ? wz : 0.f, in ? wy : 0.f, in ? wx : 0.f);
                 clear = in && ld_global(skip + f.brick) != 0;
             }
         }
