@@ -45,3 +45,38 @@ def test_served_results_equal_single_renders(hip):
             assert r.latency >= 0
     finally:
         srv.close()
+
+
+def test_served_camera_path_equals_single_renders(hip):
+    """A camera animation (CameraController moves and a CameraPath through
+    them) served as a frame stream: every frame is the image a single
+    renderer gives for that camera, and the frames come back in order."""
+    from pyvr_amd.camera import Camera, CameraController, CameraPath
+    from pyvr_amd.hip_renderer.serve import RenderServer, path_requests
+
+    vol = scenes.c3_volumes(scale=4)[1]
+    c = CameraController(scenes.standard_camera())
+    keys = [c.params.copy()]
+    c.orbit(0.6, 0.2)
+    c.zoom(1.2)
+    keys.append(c.params.copy())
+    c.trackball(dx=80, dy=-40, viewport_width=96, viewport_height=64)
+    keys.append(c.params.copy())
+    frames = CameraPath(keys).generate_frames(9)
+    base = scenes.c3_requests(1, seed=43, image_scale=8)[0]
+    srv = RenderServer(scalar_format="f16", normal_format="sn8x3", memory_budget=1 << 30, max_batch_requests=4,
+                       max_batch_pixels=1 << 16, pipeline=2)
+    try:
+        srv.register("v", vol)
+        reqs = path_requests("v", frames, base.ctf, base.otf, 96, 64, config=base.config, light=base.light,
+                             tau=base.tau, tag="path")
+        for r in reqs:
+            srv.submit(r)
+        done = srv.drain()
+        assert [r.tag[1] for r in done] == list(range(9)) and srv.batches >= 3
+        for r, cam in zip(done, frames):
+            q = scenes.Request(0, cam, base.ctf, base.otf, 96, 64, base.config, base.light, base.tau)
+            img, cnt = single(q, vol, "f16", "sn8x3")
+            assert r.rgba.tobytes() == img and cnt.sum() > 0
+    finally:
+        srv.close()
