@@ -26,7 +26,7 @@ def oracle_lib():
 
 @pytest.fixture(scope="session")
 def hip():
-    """The native library on a GPU box; fails loudly if it is missing."""
+    """The native library on a machine with a GPU; fails loudly if it is missing."""
     from pyvr_amd.hip_renderer import _lib
 
     L = _lib.lib()

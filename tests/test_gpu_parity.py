@@ -196,3 +196,29 @@ def test_non_square_ragged_frames(hip, wh):
     assert_parity(img, oracle_render(sc)["rgba8"], label=f"{w}x{h}")
     diff = np.abs(cnt.astype(np.int64) - ref["counts"].astype(np.int64))
     assert (diff == 0).mean() > 0.999 and ref["counts"].sum() > 0
+
+
+def test_pinned_read_equals_copy_read(hip):
+    """render() reads through hvr_read_pinned (one host copy from the
+    context's pinned buffer); the bytes equal hvr_read's copy into a caller
+    buffer and hvr_render's, frame after frame."""
+    import ctypes
+
+    from pyvr_amd.hip_renderer import VolumeRenderer, _lib
+
+    sc = scenes.c0(size=32, image=64)
+    r = VolumeRenderer(sc.width, sc.height, config=sc.config, light=sc.light)
+    try:
+        r.load_volume(sc.volume())
+        r.set_transfer_functions(sc.ctf, sc.otf)
+        L, ctx = _lib.lib(), r.gl_manager.ctx
+        for az in (0.3, 1.1, 2.0):
+            r.set_camera(scenes.standard_camera(azimuth=az))
+            a = r.render()
+            buf = (ctypes.c_uint8 * (sc.width * sc.height * 4))()
+            _lib.check(L.hvr_read(ctx, buf), "hvr_read")
+            out = (ctypes.c_uint8 * (sc.width * sc.height * 4))()
+            _lib.check(L.hvr_render(ctx, out), "hvr_render")
+            assert a == bytes(buf) == bytes(out) and max(a) > 0
+    finally:
+        r.close()
