@@ -126,9 +126,10 @@ def _split_appended(content: bytes):
     encoding = enc.group(1).decode() if enc else "raw"
     start = content.index(b"_", m.end()) + 1
     end = content.rindex(b"</AppendedData>")
+    # raw payloads are read by the sizes their headers give, so bytes after
+    # the last array (whitespace before the closing tag) are never used; no
+    # stripping, which could cut data bytes that happen to be whitespace
     payload = content[start:end]
-    if encoding == "raw":
-        payload = payload.rstrip(b" \t\r\n")
     xml = content[:m.end()] + content[end:]
     return xml, (encoding, payload)
 

@@ -174,3 +174,17 @@ def test_aspect_constant_and_multicomponent(tmp_path):
     write_vti(p, np.ones((2, 2, 2), np.float32), spacing=(1, 0, 1))
     with pytest.raises(ValueError, match="spacing"):
         load_vtk_volume(p)
+
+
+@pytest.mark.parametrize("zb", [0, 2])
+def test_raw_payload_ending_in_whitespace_bytes(tmp_path, zb):
+    """Raw appended data whose last bytes are 0x0a / 0x20 / 0x09 (values
+    chosen so) loads intact: the payload is read by header sizes only."""
+    d = np.zeros((2, 2, 2), np.uint32)
+    d.flat[-1] = 0x0A200A20
+    d.flat[-2] = 0x090D0A20
+    p = tmp_path / "w.vti"
+    write_vti(p, d.astype(np.int32), mode="appended-raw", zlib_blocks=zb)
+    dims, _, arrays, fmt, appended = read_vti(p)
+    vals = _array_values(arrays["Scalars_"][0], fmt, appended)
+    assert np.array_equal(vals, np.moveaxis(d.astype(np.int32), (0, 1, 2), (2, 1, 0)).ravel())
