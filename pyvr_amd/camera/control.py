@@ -348,7 +348,7 @@ class CameraController:
     def pan(self, delta_target: np.ndarray) -> None:
         if not isinstance(delta_target, np.ndarray) or delta_target.shape != (3,):
             raise ValueError("delta_target must be a 3D numpy array")
-        self.params.target = self.params.target + delta_target
+        self.params.target += delta_target           # in place: the target keeps its array and dtype
 
     def roll_camera(self, delta_roll: float) -> None:
         self.params.roll += delta_roll

@@ -81,8 +81,10 @@ def test_controller_basics():
     c.zoom(1e-6)
     assert c.params.distance == pytest.approx(0.01)                 # clamped
     t0 = c.params.target.copy()
+    held = c.params.target
     c.pan(np.array([1.0, 2.0, 3.0]))
     assert np.allclose(c.params.target, t0 + [1, 2, 3])
+    assert c.params.target is held and held.dtype == np.float32     # moved in place, as the reference does
     c.roll_camera(0.25)
     assert c.params.roll == pytest.approx(0.25)
     c.set_distance(4.0)
