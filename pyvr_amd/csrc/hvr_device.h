@@ -38,7 +38,8 @@ enum TexFmt : int {
     TF_C_F16_F32 = 7,  // float4 (nx, ny, nz, bits: half rho)     16 B
     TF_C_U8_F32 = 8,   // float4 (nx, ny, nz, bits: u8 rho)       16 B
     TF_C_F32_SN8 = 9,  // u32x2 (float rho, sn8 xyz)               8 B
-    TF_C_F16_SN8 = 10, // u32x2 (half rho, sn8 xyz)                8 B
+    TF_C_F16_SN8 = 10, // u32x2 (half rho | half qx << 16, half qy | half qz << 16), q the
+                       // SNORM8 integers in [-127, 127], exact as halves  8 B
     TF_C_U8_SN8 = 11,  // u32 sn8 xyz | u8 rho << 24               4 B
     TF_C_F16_F16 = 12, // u32x2 (half rho | half nx << 16, half ny | half nz << 16)  8 B
     TF_N_F16 = 13,     // u32x2 (half nx | half ny << 16, half nz)  8 B
@@ -162,14 +163,10 @@ __device__ __forceinline__ float4 fetch4(const char *p)
     } else if constexpr (F == TF_C_F32_SN8) {
         const uint2 t = *(const uint2 *)p;
         return make_float4(sn8f(t.y, 0), sn8f(t.y, 8), sn8f(t.y, 16), __uint_as_float(t.x));
-    } else if constexpr (F == TF_C_F16_SN8) {
-        const uint2 t = *(const uint2 *)p;
-        const __half h = __ushort_as_half((unsigned short)(t.x & 0xffffu));
-        return make_float4(sn8f(t.y, 0), sn8f(t.y, 8), sn8f(t.y, 16), __half2float(h));
     } else if constexpr (F == TF_C_U8_SN8) {
         const uint32_t w = *(const uint32_t *)p;
         return make_float4(sn8f(w, 0), sn8f(w, 8), sn8f(w, 16), (float)(w >> 24));
-    } else if constexpr (F == TF_C_F16_F16) {
+    } else if constexpr (F == TF_C_F16_F16 || F == TF_C_F16_SN8) {
         const uint2 t = *(const uint2 *)p;
         return make_float4(h16f(t.x >> 16), h16f(t.y), h16f(t.y >> 16), h16f(t.x));
     } else if constexpr (F == TF_N_F16) {
@@ -366,12 +363,9 @@ __device__ __forceinline__ float4 decode_words(const uint32_t *w)
                            (float)(w[3] & 0xffu));
     } else if constexpr (F == TF_C_F32_SN8) {
         return make_float4(sn8f(w[1], 0), sn8f(w[1], 8), sn8f(w[1], 16), __uint_as_float(w[0]));
-    } else if constexpr (F == TF_C_F16_SN8) {
-        return make_float4(sn8f(w[1], 0), sn8f(w[1], 8), sn8f(w[1], 16),
-                           __half2float(__ushort_as_half((unsigned short)(w[0] & 0xffffu))));
     } else if constexpr (F == TF_C_U8_SN8) {
         return make_float4(sn8f(w[0], 0), sn8f(w[0], 8), sn8f(w[0], 16), (float)(w[0] >> 24));
-    } else if constexpr (F == TF_C_F16_F16) {
+    } else if constexpr (F == TF_C_F16_F16 || F == TF_C_F16_SN8) {
         return make_float4(h16f(w[0] >> 16), h16f(w[1]), h16f(w[1] >> 16), h16f(w[0]));
     } else if constexpr (F == TF_N_F16) {
         const float x = h16f(w[0]);
@@ -389,9 +383,8 @@ __device__ __forceinline__ float decode_scalar(const uint32_t *w)
     else if constexpr (F == TF_C_F16_F32) return __half2float(__ushort_as_half((unsigned short)(w[3] & 0xffffu)));
     else if constexpr (F == TF_C_U8_F32) return (float)(w[3] & 0xffu);
     else if constexpr (F == TF_C_F32_SN8) return __uint_as_float(w[0]);
-    else if constexpr (F == TF_C_F16_SN8) return __half2float(__ushort_as_half((unsigned short)(w[0] & 0xffffu)));
     else if constexpr (F == TF_C_U8_SN8) return (float)(w[0] >> 24);
-    else if constexpr (F == TF_C_F16_F16) return h16f(w[0]);
+    else if constexpr (F == TF_C_F16_F16 || F == TF_C_F16_SN8) return h16f(w[0]);
     else return 0.f;
 }
 template <int F>
@@ -399,11 +392,11 @@ __device__ __forceinline__ float3 decode_normal(const uint32_t *w)
 {
     if constexpr (F == TF_C_F32_F32 || F == TF_C_F16_F32 || F == TF_C_U8_F32)
         return make_float3(__uint_as_float(w[0]), __uint_as_float(w[1]), __uint_as_float(w[2]));
-    else if constexpr (F == TF_C_F32_SN8 || F == TF_C_F16_SN8)
+    else if constexpr (F == TF_C_F32_SN8)
         return make_float3(sn8f(w[1], 0), sn8f(w[1], 8), sn8f(w[1], 16));
     else if constexpr (F == TF_C_U8_SN8)
         return make_float3(sn8f(w[0], 0), sn8f(w[0], 8), sn8f(w[0], 16));
-    else if constexpr (F == TF_C_F16_F16)
+    else if constexpr (F == TF_C_F16_F16 || F == TF_C_F16_SN8)
         return make_float3(h16f(w[0] >> 16), h16f(w[1]), h16f(w[1] >> 16));
     else return make_float3(0.f, 0.f, 0.f);
 }
@@ -1051,9 +1044,8 @@ __device__ __forceinline__ void store_part(char *p, int part, const float *val)
         else if constexpr (F == TF_C_F16_F32) ((uint32_t *)p)[3] = (uint32_t)__half_as_ushort(__float2half_rn(v));
         else if constexpr (F == TF_C_U8_F32) ((uint32_t *)p)[3] = enc_u8(v);
         else if constexpr (F == TF_C_F32_SN8) ((float *)p)[0] = v;
-        else if constexpr (F == TF_C_F16_SN8) ((uint32_t *)p)[0] = (uint32_t)__half_as_ushort(__float2half_rn(v));
         else if constexpr (F == TF_C_U8_SN8) ((uint8_t *)p)[3] = (uint8_t)enc_u8(v);
-        else if constexpr (F == TF_C_F16_F16) *(__half *)p = __float2half_rn(v);
+        else if constexpr (F == TF_C_F16_F16 || F == TF_C_F16_SN8) *(__half *)p = __float2half_rn(v);
     } else {
         const float nx = val[0], ny = val[1], nz = val[2];
         const uint32_t packed = enc_sn8(nx) | (enc_sn8(ny) << 8) | (enc_sn8(nz) << 16);
@@ -1063,8 +1055,15 @@ __device__ __forceinline__ void store_part(char *p, int part, const float *val)
             if constexpr (F == TF_N_F32) q[3] = 0.f;
         } else if constexpr (F == TF_N_SN8) {
             *(uint32_t *)p = packed;
-        } else if constexpr (F == TF_C_F32_SN8 || F == TF_C_F16_SN8) {
+        } else if constexpr (F == TF_C_F32_SN8) {
             ((uint32_t *)p)[1] = packed;
+        } else if constexpr (F == TF_C_F16_SN8) {
+            // the SNORM8 integers as halves (exact), which the march feeds
+            // to v_fma_mix without a conversion per channel and tap
+            __half *q = (__half *)p;   // halves 1..3; half 0 is the scalar
+            q[1] = __int2half_rn((int)(int8_t)(packed & 0xffu));
+            q[2] = __int2half_rn((int)(int8_t)((packed >> 8) & 0xffu));
+            q[3] = __int2half_rn((int)(int8_t)((packed >> 16) & 0xffu));
         } else if constexpr (F == TF_C_U8_SN8) {
             uint8_t *q = (uint8_t *)p;
             q[0] = (uint8_t)(packed & 0xffu); q[1] = (uint8_t)((packed >> 8) & 0xffu);
