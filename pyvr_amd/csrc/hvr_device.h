@@ -849,11 +849,16 @@ __device__ __forceinline__ void march_ray(const FrameConst &fc, const TexDesc &t
             }
         } else {
             // The in-box run with transparent bricks stepped over: a sample
-            // in a transparent brick is counted without loading texels.
+            // in a transparent brick is counted without loading texels.  The
+            // flag is loaded again only when the sample's brick changes (a
+            // brick spans several steps), which saves most flag loads and
+            // their waits.
             float wx = x - fc.bmin[0], wy = y - fc.bmin[1], wz = z - fc.bmin[2];
             bool in = i < nlim && inbox(fc, wx, wy, wz);
             Footprint f = footprint_brick<LB, BUF>(ts, in ? wz : 0.f, in ? wy : 0.f, in ? wx : 0.f);
-            bool clear = in && ld_global(skip + f.brick) != 0;
+            uint32_t cur = f.brick;
+            bool cflag = ld_global(skip + cur) != 0;
+            bool clear = in && cflag;
             while (in && A < fc.tau) {
                 if (!clear) shade(f);
                 ++i;
@@ -866,7 +871,11 @@ __device__ __forceinline__ void march_ray(const FrameConst &fc, const TexDesc &t
                 wz = z - fc.bmin[2];
                 in = i < nlim && inbox(fc, wx, wy, wz);
                 f = footprint_brick<LB, BUF>(ts, in ? wz : 0.f, in ? wy : 0.f, in ? wx : 0.f);
-                clear = in && ld_global(skip + f.brick) != 0;
+                if (f.brick != cur) {
+                    cur = f.brick;
+                    cflag = ld_global(skip + cur) != 0;
+                }
+                clear = in && cflag;
             }
         }
     }
