@@ -835,27 +835,46 @@ __device__ __forceinline__ void march_ray(const FrameConst &fc, const TexDesc &t
                 composite<NM>(fc, q, NM == NM_ALIAS ? rho : 0.f, 0.f, 0.f, C0, C1, C2, A);
             }
         };
+        // The in-box run tests one bound per axis.  Per axis, with s = -1
+        // where the step is negative (else +1), the run walks q = s*p by
+        // s*ds >= 0 and forms a = fl(q - s*bmin): round-to-nearest commutes
+        // with negation, so q is exactly s times the shader's position and
+        // a = s*fl(p - bmin).  fl is monotonic, so a never decreases; the
+        // run starts at a sample that passed the two-sided test, so a's
+        // lower bound stays met and only a <= (s > 0 ? ext : 0) is tested.
+        // In the box, p - bmin >= 0, so the footprint takes |a| (a free
+        // source modifier) as fl(p - bmin).
+        const float sx = r.dsx < 0.f ? -1.f : 1.f, sy = r.dsy < 0.f ? -1.f : 1.f,
+                    sz = r.dsz < 0.f ? -1.f : 1.f;
+        x *= sx;
+        y *= sy;
+        z *= sz;
+        const float dqx = sx * r.dsx, dqy = sy * r.dsy, dqz = sz * r.dsz;
+        const float bqx = -(sx * fc.bmin[0]), bqy = -(sy * fc.bmin[1]), bqz = -(sz * fc.bmin[2]);
+        const float lx = sx > 0.f ? fc.ext[0] : 0.f, ly = sy > 0.f ? fc.ext[1] : 0.f,
+                    lz = sz > 0.f ? fc.ext[2] : 0.f;
         if constexpr (!SKIP) {
             // the in-box run
             for (; i < nlim; ++i) {
                 if (!(A < fc.tau)) break;
-                const float wx = x - fc.bmin[0], wy = y - fc.bmin[1], wz = z - fc.bmin[2];
-                if (!inbox(fc, wx, wy, wz)) break;
+                const float ax = __fadd_rn(x, bqx), ay = __fadd_rn(y, bqy), az = __fadd_rn(z, bqz);
+                if (!((az <= lz) & (ay <= ly) & (ax <= lx))) break;
                 if constexpr (COUNT) ++ns;
-                shade(footprint_brick<LB, BUF>(ts, wz, wy, wx));
-                x = __fadd_rn(x, r.dsx);
-                y = __fadd_rn(y, r.dsy);
-                z = __fadd_rn(z, r.dsz);
+                shade(footprint_brick<LB, BUF>(ts, fabsf(az), fabsf(ay), fabsf(ax)));
+                x = __fadd_rn(x, dqx);
+                y = __fadd_rn(y, dqy);
+                z = __fadd_rn(z, dqz);
             }
         } else {
             // The in-box run with transparent bricks stepped over: a sample
             // in a transparent brick is counted without loading texels.  The
             // flag is loaded again only when the sample's brick changes (a
             // brick spans several steps), which saves most flag loads and
-            // their waits.
-            float wx = x - fc.bmin[0], wy = y - fc.bmin[1], wz = z - fc.bmin[2];
-            bool in = i < nlim && inbox(fc, wx, wy, wz);
-            Footprint f = footprint_brick<LB, BUF>(ts, in ? wz : 0.f, in ? wy : 0.f, in ? wx : 0.f);
+            // their waits.  A footprint outside the box is never used.
+            float ax = __fadd_rn(x, bqx), ay = __fadd_rn(y, bqy), az = __fadd_rn(z, bqz);
+            bool in = i < nlim && (az <= lz) & (ay <= ly) & (ax <= lx);
+            Footprint f = footprint_brick<LB, BUF>(ts, in ? fabsf(az) : 0.f, in ? fabsf(ay) : 0.f,
+                                                   in ? fabsf(ax) : 0.f);
             uint32_t cur = f.brick;
             bool cflag = ld_global(skip + cur) != 0;
             bool clear = in && cflag;
@@ -863,15 +882,15 @@ __device__ __forceinline__ void march_ray(const FrameConst &fc, const TexDesc &t
                 if (!clear) shade(f);
                 ++i;
                 if constexpr (COUNT) ++ns;
-                x = __fadd_rn(x, r.dsx);
-                y = __fadd_rn(y, r.dsy);
-                z = __fadd_rn(z, r.dsz);
-                wx = x - fc.bmin[0];
-                wy = y - fc.bmin[1];
-                wz = z - fc.bmin[2];
-                in = i < nlim && inbox(fc, wx, wy, wz);
-                f = footprint_brick<LB, BUF>(ts, in ? wz : 0.f, in ? wy : 0.f, in ? wx : 0.f);
-                if (f.brick != cur) {
+                x = __fadd_rn(x, dqx);
+                y = __fadd_rn(y, dqy);
+                z = __fadd_rn(z, dqz);
+                ax = __fadd_rn(x, bqx);
+                ay = __fadd_rn(y, bqy);
+                az = __fadd_rn(z, bqz);
+                in = i < nlim && (az <= lz) & (ay <= ly) & (ax <= lx);
+                f = footprint_brick<LB, BUF>(ts, fabsf(az), fabsf(ay), fabsf(ax));
+                if (in && f.brick != cur) {
                     cur = f.brick;
                     cflag = ld_global(skip + cur) != 0;
                 }
