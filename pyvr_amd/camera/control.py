@@ -28,7 +28,7 @@ from typing import List, Optional, Tuple
 
 import numpy as np
 
-from .camera import Camera, validate_camera_angles
+from .camera import Camera, _cross3, _norm, validate_camera_angles
 
 
 def _quat_from_rotvec(v) -> np.ndarray:
@@ -49,7 +49,7 @@ def _quat_mul(p: np.ndarray, q: np.ndarray) -> np.ndarray:
     """Hamilton product p*q of (x, y, z, w) quaternions: rotate by q, then by p."""
     pv, pw = p[:3], p[3]
     qv, qw = q[:3], q[3]
-    v = pw * qv + qw * pv + np.cross(pv, qv)
+    v = pw * qv + qw * pv + _cross3(pv, qv)
     return np.array([v[0], v[1], v[2], pw * qw - pv @ qv])
 
 
@@ -101,21 +101,21 @@ def get_camera_pos(
     init_up = init_up.astype(np.float32)
 
     offset = init_pos - target
-    norm = np.linalg.norm(offset)
+    norm = _norm(offset)
     if norm == 0:
         raise ValueError("init_pos must not be the zero vector (relative to target)")
     offset = offset / norm * distance
 
     q_az = _quat_from_rotvec(azimuth * init_up)
 
-    view_dir = -offset / np.linalg.norm(offset)
-    axis = np.cross(init_up, view_dir)
-    axis_norm = np.linalg.norm(axis)
+    view_dir = -offset / _norm(offset)
+    axis = _cross3(init_up, view_dir)
+    axis_norm = _norm(axis)
     if axis_norm < 1e-6:
         # init_up parallel to the view direction: pick a fixed helper axis
         helper = np.array([1, 0, 0]) if abs(init_up[0]) < 0.9 else np.array([0, 0, 1])
-        axis = np.cross(init_up, helper)
-        axis_norm = np.linalg.norm(axis)
+        axis = _cross3(init_up, helper)
+        axis_norm = _norm(axis)
     axis = axis / axis_norm
     q_el = _quat_from_rotvec(elevation * axis)
     q_roll = _quat_from_rotvec(roll * view_dir)

@@ -67,6 +67,7 @@ struct hvr_ctx {
     float *d_stage;  // upload staging
     size_t stage_bytes;
     hipEvent_t ev[4];
+    hipEvent_t rev[4];   // readback chunks (hvr_read)
     bool events_recorded;
     hvr_params params;
     bool params_set;
@@ -446,6 +447,7 @@ int hvr_create(int device, int width, int height, hvr_ctx **out)
     if (e == hipSuccess) e = hipMalloc(&c->d_rays, npix * sizeof(RayRec));
     if (e == hipSuccess) e = hipHostMalloc(&c->h_pinned, npix * 4, hipHostMallocDefault);
     for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
+    for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&c->rev[i], hipEventDisableTiming);
     if (e == hipSuccess) {
         // unbound LUT: a 1-texel (0, 0, 0, 1) texture (GL incomplete texture)
         c->lut_cap = 1;
@@ -477,8 +479,10 @@ int hvr_destroy(hvr_ctx *c)
     (void)hipFree(c->d_flags);
     (void)hipFree(c->d_nclear);
     if (c->h_pinned) (void)hipHostFree(c->h_pinned);
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 4; ++i) {
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+        if (c->rev[i]) (void)hipEventDestroy(c->rev[i]);
+    }
     if (c->tev) {
         for (hipEvent_t e : *c->tev) (void)hipEventDestroy(e);
         delete c->tev;
@@ -967,14 +971,29 @@ int hvr_timing_end(hvr_ctx *c, double *march_ms, double *setup_ms, int *frames)
     return HVR_OK;
 }
 
+// The copy runs in four chunks through the pinned buffer: each chunk's
+// host copy into `out` starts as soon as its device-to-host copy has
+// landed, overlapping the transfer of the next.
 int hvr_read(hvr_ctx *c, uint8_t *out)
 {
     if (!c || !out) return fail(HVR_EINVAL, "null argument");
     HIPCHK(hipSetDevice(c->device));
     const size_t bytes = (size_t)c->W * c->H * 4;
-    HIPCHK(hipMemcpyAsync(c->h_pinned, c->d_out, bytes, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    memcpy(out, c->h_pinned, bytes);
+    constexpr int K = 4;
+    const size_t chunk = ((bytes + K - 1) / K + 4095) & ~(size_t)4095;
+    size_t off[K + 1];
+    for (int k = 0; k <= K; ++k) off[k] = std::min(bytes, (size_t)k * chunk);
+    uint8_t *h = (uint8_t *)c->h_pinned;
+    const uint8_t *d = (const uint8_t *)c->d_out;
+    for (int k = 0; k < K; ++k) {
+        if (off[k + 1] > off[k])
+            HIPCHK(hipMemcpyAsync(h + off[k], d + off[k], off[k + 1] - off[k], hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipEventRecord(c->rev[k], c->stream));
+    }
+    for (int k = 0; k < K; ++k) {
+        HIPCHK(hipEventSynchronize(c->rev[k]));
+        memcpy(out + off[k], h + off[k], off[k + 1] - off[k]);
+    }
     return HVR_OK;
 }
 

@@ -36,6 +36,16 @@ from ._lib import HvrParams, check
 # Uniforms of the march program (reference volume.frag.glsl:3-19), plus the
 # early-termination alpha that the reference hard-codes at 0.99 (:87).
 _SAMPLERS = ("volume_texture", "normal_volume", "transfer_function_lut")
+
+# A bytes object of a given size whose contents the library fills before it
+# is returned (the CPython C API allows writing a new bytes object's buffer
+# until it is shared).
+_new_bytes = ctypes.pythonapi.PyBytes_FromStringAndSize
+_new_bytes.restype = ctypes.py_object
+_new_bytes.argtypes = (ctypes.c_void_p, ctypes.c_ssize_t)
+_bytes_data = ctypes.pythonapi.PyBytes_AsString
+_bytes_data.restype = ctypes.c_void_p
+_bytes_data.argtypes = (ctypes.py_object,)
 _UNIFORMS = {
     "volume_texture": "int", "normal_volume": "int", "transfer_function_lut": "int",
     "view_matrix": "mat4", "projection_matrix": "mat4", "camera_pos": "vec3",
@@ -404,11 +414,14 @@ class HipManager:
         check(L.hvr_render_async(self.ctx), "hvr_render_async")
 
     def read_pixels(self) -> bytes:
-        """Framebuffer contents: width*height*4 bytes, RGBA8, bottom row first
-        (copied once, from the context's pinned host buffer)."""
-        ptr = ctypes.POINTER(ctypes.c_uint8)()
-        check(self.L.hvr_read_pinned(self.ctx, ctypes.byref(ptr)), "hvr_read_pinned")
-        return ctypes.string_at(ptr, self.width * self.height * 4)
+        """Framebuffer contents: width*height*4 bytes, RGBA8, bottom row first.
+        The library copies the image straight into a new bytes object
+        (hvr_read: chunked through pinned memory, each chunk's host copy
+        overlapping the next chunk's transfer), so no further copy is made."""
+        n = self.width * self.height * 4
+        out = _new_bytes(None, n)
+        check(self.L.hvr_read(self.ctx, ctypes.cast(_bytes_data(out), ctypes.POINTER(ctypes.c_uint8))), "hvr_read")
+        return out
 
     def sample_counts(self) -> np.ndarray:
         """Samples composited per pixel for the current state (H, W) uint32,

@@ -107,9 +107,10 @@ class HipVolumeRenderer:
             raise TypeError(f"Expected Camera instance, got {type(camera)}")
         self.camera = camera
         aspect = self.width / self.height
-        view_matrix = camera.get_view_matrix()
+        # the camera vectors once: get_view_matrix() would compute them again
+        position, up = camera.get_camera_vectors()
+        view_matrix = camera._view_matrix(position, up)
         projection_matrix = camera.get_projection_matrix(aspect)
-        position, _ = camera.get_camera_vectors()
         self.gl_manager.set_uniform_matrix("view_matrix", view_matrix)
         self.gl_manager.set_uniform_matrix("projection_matrix", projection_matrix)
         self.gl_manager.set_uniform_vector("camera_pos", tuple(position))

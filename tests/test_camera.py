@@ -162,3 +162,29 @@ def test_presets_and_serialisation(tmp_path):
     assert "azimuth" in repr(iso)
     assert degrees_to_radians(a=180.0)["a"] == pytest.approx(np.pi)
     assert radians_to_degrees(a=np.pi)["a"] == pytest.approx(180.0)
+
+
+@pytest.mark.parametrize("da,db", [(np.float32, np.float32), (np.float64, np.float64),
+                                   (np.float32, np.float64), (np.float64, np.float32)])
+def test_vector_helpers_bitwise_equal_numpy(da, db):
+    """The per-frame camera path's 3-vector cross product and norm give
+    np.cross's and np.linalg.norm's results bit for bit, dtype included."""
+    from pyvr_amd.camera.camera import _cross3, _norm
+    rng = np.random.default_rng(5)
+    for _ in range(300):
+        a = (rng.normal(size=3) * 10.0 ** rng.integers(-8, 8)).astype(da)
+        b = (rng.normal(size=3) * 10.0 ** rng.integers(-8, 8)).astype(db)
+        c, ref = _cross3(a, b), np.cross(a, b)
+        assert c.dtype == ref.dtype and np.array_equal(c.view(np.uint8), ref.view(np.uint8))
+        n, nref = _norm(a), np.linalg.norm(a)
+        assert n.dtype == nref.dtype and n.tobytes() == nref.tobytes()
+
+
+def test_view_matrix_from_vectors_equals_get_view_matrix():
+    rng = np.random.default_rng(6)
+    for _ in range(50):
+        cam = Camera.from_spherical(target=rng.normal(size=3).astype(np.float32), azimuth=rng.uniform(-3, 3),
+                                    elevation=rng.uniform(-1.5, 1.5), roll=rng.uniform(-3, 3),
+                                    distance=rng.uniform(0.5, 10))
+        pos, up = cam.get_camera_vectors()
+        assert cam._view_matrix(pos, up).tobytes() == cam.get_view_matrix().tobytes()

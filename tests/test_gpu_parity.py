@@ -199,26 +199,30 @@ def test_non_square_ragged_frames(hip, wh):
 
 
 def test_pinned_read_equals_copy_read(hip):
-    """render() reads through hvr_read_pinned (one host copy from the
-    context's pinned buffer); the bytes equal hvr_read's copy into a caller
-    buffer and hvr_render's, frame after frame."""
+    """render() fills its bytes through hvr_read (chunked through pinned
+    memory); they equal hvr_read_pinned's buffer and hvr_render's copy, frame
+    after frame, at sizes whose byte counts split into whole, ragged and
+    empty chunks."""
     import ctypes
 
     from pyvr_amd.hip_renderer import VolumeRenderer, _lib
 
-    sc = scenes.c0(size=32, image=64)
-    r = VolumeRenderer(sc.width, sc.height, config=sc.config, light=sc.light)
-    try:
-        r.load_volume(sc.volume())
-        r.set_transfer_functions(sc.ctf, sc.otf)
-        L, ctx = _lib.lib(), r.gl_manager.ctx
-        for az in (0.3, 1.1, 2.0):
-            r.set_camera(scenes.standard_camera(azimuth=az))
-            a = r.render()
-            buf = (ctypes.c_uint8 * (sc.width * sc.height * 4))()
-            _lib.check(L.hvr_read(ctx, buf), "hvr_read")
-            out = (ctypes.c_uint8 * (sc.width * sc.height * 4))()
-            _lib.check(L.hvr_render(ctx, out), "hvr_render")
-            assert a == bytes(buf) == bytes(out) and max(a) > 0
-    finally:
-        r.close()
+    for w, h in ((64, 64), (200, 170), (13, 7)):
+        sc = scenes.c0(size=32, image=64)
+        r = VolumeRenderer(w, h, config=sc.config, light=sc.light)
+        try:
+            r.load_volume(sc.volume())
+            r.set_transfer_functions(sc.ctf, sc.otf)
+            L, ctx = _lib.lib(), r.gl_manager.ctx
+            for az in (0.3, 1.1, 2.0):
+                r.set_camera(scenes.standard_camera(azimuth=az))
+                a = r.render()
+                ptr = ctypes.POINTER(ctypes.c_uint8)()
+                _lib.check(L.hvr_read_pinned(ctx, ctypes.byref(ptr)), "hvr_read_pinned")
+                pinned = ctypes.string_at(ptr, w * h * 4)
+                out = (ctypes.c_uint8 * (w * h * 4))()
+                _lib.check(L.hvr_render(ctx, out), "hvr_render")
+                assert isinstance(a, bytes) and len(a) == w * h * 4
+                assert a == pinned == bytes(out) and max(a) > 0, (w, h, az)
+        finally:
+            r.close()
