@@ -971,17 +971,20 @@ int hvr_timing_end(hvr_ctx *c, double *march_ms, double *setup_ms, int *frames)
     return HVR_OK;
 }
 
-// The copy runs in four chunks through the pinned buffer: each chunk's
-// host copy into `out` starts as soon as its device-to-host copy has
-// landed, overlapping the transfer of the next.
+// The copy runs through the pinned buffer in chunks of at least 1 MiB (at
+// most four): each chunk's host copy into `out` starts as soon as its
+// device-to-host copy has landed, overlapping the transfer of the next.
+// Images up to 1 MiB take one copy, since the per-chunk synchronisation
+// costs more than the overlap saves there.
 int hvr_read(hvr_ctx *c, uint8_t *out)
 {
     if (!c || !out) return fail(HVR_EINVAL, "null argument");
     HIPCHK(hipSetDevice(c->device));
     const size_t bytes = (size_t)c->W * c->H * 4;
-    constexpr int K = 4;
+    constexpr int KMAX = 4;
+    const int K = (int)std::min<size_t>(KMAX, std::max<size_t>(1, bytes >> 20));
     const size_t chunk = ((bytes + K - 1) / K + 4095) & ~(size_t)4095;
-    size_t off[K + 1];
+    size_t off[KMAX + 1];
     for (int k = 0; k <= K; ++k) off[k] = std::min(bytes, (size_t)k * chunk);
     uint8_t *h = (uint8_t *)c->h_pinned;
     const uint8_t *d = (const uint8_t *)c->d_out;

@@ -199,15 +199,15 @@ def test_non_square_ragged_frames(hip, wh):
 
 
 def test_pinned_read_equals_copy_read(hip):
-    """render() fills its bytes through hvr_read (chunked through pinned
-    memory); they equal hvr_read_pinned's buffer and hvr_render's copy, frame
-    after frame, at sizes whose byte counts split into whole, ragged and
-    empty chunks."""
+    """render() fills its bytes through hvr_read (through pinned memory, in
+    chunks of at least 1 MiB); they equal hvr_read_pinned's buffer and
+    hvr_render's copy, frame after frame, at sizes taking one copy and
+    two or three chunks (the last one ragged)."""
     import ctypes
 
     from pyvr_amd.hip_renderer import VolumeRenderer, _lib
 
-    for w, h in ((64, 64), (200, 170), (13, 7)):
+    for w, h in ((64, 64), (200, 170), (13, 7), (1024, 600), (1000, 1000)):
         sc = scenes.c0(size=32, image=64)
         r = VolumeRenderer(w, h, config=sc.config, light=sc.light)
         try:
