@@ -120,6 +120,23 @@ def test_batch_formats_bricks_and_no_normals(hip, sf, nf, lb, normals):
         b.close()
 
 
+def test_batch_mixed_classes(hip):
+    """Volumes with and without normals in one batch (two kernel
+    instances, so two classes launched in the same graph), interleaved in
+    request order: every request equals its single render."""
+    import dataclasses
+
+    vols = scenes.c3_volumes(scale=4, with_normals=True) + scenes.c3_volumes(scale=4, with_normals=False)
+    reqs = scenes.c3_requests(8, seed=31, image_scale=4)
+    reqs = [dataclasses.replace(q, volume=q.volume + (4 if k % 2 else 0)) for k, q in enumerate(reqs)]
+    b, idx = make_batch(vols, reqs, scalar_format="f16", normal_format="sn8x3")
+    try:
+        b.run(count=True)
+        check_against_single(b, idx, vols, reqs, "f16", "sn8x3")
+    finally:
+        b.close()
+
+
 def test_batch_global_load_instances(hip, monkeypatch):
     """The 64-bit-load batch kernels (volumes of 4 GiB and more; forced here
     with HVR_GATHER=global) give the same bytes and counts as the
