@@ -2,42 +2,50 @@
 """Benchmark of the HIP volume renderer on one MI355X.
 
 Metric: ray samples/s (plus fps and p50/p99 latency) at 512^3 / 1024^2,
-batched form against the plain one-request-per-launch kernel.  A *step*
-renders the workload's camera set once (default: 24 cameras of the 512^3
-blob volume at 1024x1024, high_quality preset, FP16 scalar + SNORM8x3
-normals), with the volume already resident in HBM.
+batched and served forms against the plain one-request-per-launch kernel.
 
-* plain leg: each frame one launch (ray setup + march) into its own resident
-  device buffer, back to back on one stream; its figures are the
-  ``plain_kernel`` object of the line (and the whole line for c0 / c1, or
-  with --plain-only);
-* batched leg (c2): the 24 cameras as 24 requests of one persistent
+Default run (what one ``python bench.py`` prints, one JSON line):
+
+* workload: the 512^3 Gaussian-blob volume (32 blobs, seed 2, generated on
+  the device), FP16 scalars + FP16x3 normals, 24 cameras on a sphere of
+  radius 3 at 1024x1024, high_quality preset, termination at alpha 0.95;
+  the volume is resident in HBM before any timed region;
+* a *step* renders the 24 cameras once;
+* batched leg (``value``): the 24 cameras as 24 requests of one persistent
   graph-captured launch per step (BatchRenderer; brick size and ray tile
   from the autotuner), two steps in flight, every step ending with its 24
-  images in pinned host memory; ``value`` is this leg's;
+  images in pinned host memory;
+* plain leg (``plain_kernel``): the same frames one launch (ray setup +
+  march) per frame -- the single-frame kernels VolumeRenderer.render()
+  runs -- back to back into resident device buffers; ``frame_ms_p50/p99``
+  are end-to-end per-frame calls (camera update, draw, RGBA8 copy to the
+  host);
+* serving leg (``serving``): 16 resident device-generated volumes of 256^3
+  to 1024^3, Poisson arrivals at 500 and 1000 req/s, images of 512^2 to
+  2048^2 over the fast / balanced / high_quality presets; p50/p99 latency
+  (arrival to pixels in host memory) and samples/s against the plain
+  one-request-per-launch renderer on the same requests;
 * samples: reference-algorithm samples (pixel, i) -- i < max_steps, alpha
   below the termination threshold, sample inside the box -- counted by the
   renderer's counting pass before the timed region (the tests pin that count
   to the oracle's);
-* value = samples per step * steps / wall time of the timed steps;
-* roofline: the dominant kernel (k_batch for the batched leg, k_march for
-  the plain one) against HBM peak; algorithmic bytes per launch = samples *
-  8 * (scalar + normal bytes per voxel, unpadded) + 4 * W * H per image,
-  divided by the launch's average duration from HIP events recorded on its
-  stream (for the batched leg: around the graph, which also holds the
-  staging copy, a counter fill and the brick-flag kernel).  Most texel reads
-  hit L1/L2, so that rate exceeds HBM peak; ``traffic`` and
-  ``hbm_measured_*`` give the HBM bytes rocprofv3 measured for these
-  sources, and ``profile`` the kernel's VALU-busy fraction (the limit);
-* fps / latency: batched frames per second; ``batch_ms_p50/p99``: one step
-  of the batch from launch to images in host memory; ``frame_ms_p50/p99``:
-  end-to-end VolumeRenderer.render() (set_camera + draw + copy of the RGBA8
-  image to the host) per frame;
+* roofline of the dominant kernel (the batched launch): ``traffic`` is the
+  HBM bytes per launch that rocprofv3 measured for these kernel sources
+  (FETCH_SIZE and WRITE_SIZE passes, scaled by a copy-kernel calibration;
+  profiles/traffic.json), ``achieved`` that over the launch's HIP-event
+  duration, ``frac`` against 8 TB/s.  ``alg`` is the algorithmic figure (8
+  texel reads per reference sample at the stored format, plus the images),
+  which counts L1/L2 re-reads and exceeds HBM peak; ``valu`` is the
+  instruction-issue roof that binds the kernel (VALU instructions per sample
+  from the SQ counter pass, over 1024 SIMDs issuing one wave64 instruction
+  per 4 cycles at 2.4 GHz);
 * cpu_baseline: the oracle's FP32 port of the reference shader (its loop
-  runs every i < max_steps, as the shader does) on all host cores, whole
-  frames of the workload's cameras until ~15 s have elapsed.
+  runs every i < max_steps, as the shader does) on all host cores, on the
+  volume's stored values read back from the device, whole frames of the
+  workload's cameras for about 12 s.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+       python bench.py --workload c0|c1|c3|c4 (single workloads)
 """
 
 import argparse
@@ -53,7 +61,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
-HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
+VALU_PEAK = 1024 * 2.4e9 / 4 * 64   # lane instructions/s: 1024 SIMDs, wave64 per 4 cycles, 2.4 GHz
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 SCALAR_BYTES = {"f32": 4, "f16": 2, "u8": 1}
 NORMAL_BYTES = {"f32x3": 12, "sn8x3": 3, "f16x3": 6, "none": 0}
@@ -64,22 +73,57 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_workload(args):
+# ---------------------------------------------------------------------------
+# workloads
+
+class Workload:
+    """Everything a frame of the workload needs besides the camera."""
+
+    def __init__(self, name, volume, ctf, otf, config, light, tau, width, height, cams):
+        self.name, self.volume, self.ctf, self.otf = name, volume, ctf, otf
+        self.config, self.light, self.tau = config, light, tau
+        self.width, self.height, self.cams = width, height, cams
+
+    @property
+    def shape(self):
+        data = getattr(self.volume, "data", None)
+        return tuple(data.shape) if data is not None else tuple(self.volume.shape)
+
+    @property
+    def has_normals(self):
+        return bool(self.volume.has_normals)
+
+    def requests(self):
+        import scenes
+
+        return [scenes.Request(0, cam, self.ctf, self.otf, self.width, self.height, self.config, self.light,
+                               self.tau) for cam in self.cams]
+
+
+def make_workload(args) -> Workload:
     import scenes
+    from pyvr_amd.config import RenderConfig
+    from pyvr_amd.hip_renderer.batch import DeviceBlobVolume
+    from pyvr_amd.transferfunctions import ColorTransferFunction
 
     if args.workload == "c2":
-        sc = scenes.c2(size=args.size or 512, image=args.image or 1024)
-        cams = scenes.c2_cameras(args.cameras)
-    elif args.workload == "c1":
+        n, img = args.size or 512, args.image or 1024
+        vol = DeviceBlobVolume(shape=(n, n, n), n_blobs=32, seed=2, noise=1e-3, with_normals=True)
+        return Workload(f"c2dev_blobs32_{n}_{img}", vol, ColorTransferFunction.from_colormap("viridis"),
+                        scenes.seeded_otf(6, seed=2), RenderConfig.high_quality(), scenes.standard_light(), 0.95,
+                        img, img, scenes.c2_cameras(args.cameras))
+    if args.workload == "c1":
         sc = scenes.c1(size=args.size or 256, image=args.image or 1024)
-        cams = [sc.camera] * args.cameras
     elif args.workload == "c0":
         sc = scenes.c0(size=args.size or 128, image=args.image or 512)
-        cams = [sc.camera] * args.cameras
     else:
         raise SystemExit(f"unknown workload {args.workload}")
-    return sc, cams
+    return Workload(sc.name, sc.volume(), sc.ctf, sc.otf, sc.config, sc.light, sc.tau, sc.width, sc.height,
+                    [sc.camera] * args.cameras)
 
+
+# ---------------------------------------------------------------------------
+# roofline
 
 def kernel_source_hash() -> str:
     """Hash of the device sources: measured traffic is only reported for the
@@ -87,247 +131,192 @@ def kernel_source_hash() -> str:
     import hashlib
 
     h = hashlib.sha256()
-    for rel in ("pyvr_amd/csrc/hvr_device.h", "pyvr_amd/csrc/hipvr.hip"):
+    for rel in ("pyvr_amd/csrc/hvr_device.h", "pyvr_amd/csrc/hvr_rules.h", "pyvr_amd/csrc/hipvr.hip"):
         with open(os.path.join(ROOT, rel), "rb") as f:
             h.update(f.read())
     return h.hexdigest()[:16]
 
 
-def measured_traffic(key: str):
-    """HBM bytes per launch from the rocprofv3 FETCH_SIZE/WRITE_SIZE passes
-    recorded in profiles/traffic.json (scaled by the copy-kernel
-    calibration), with the profile's other figures of that kernel (VALU
-    busy, resident waves, L2 hit rate), or (None, None, {}) when no
-    measurement matches this workload and these kernel sources."""
+def measured_profile(key: str) -> dict:
+    """The counter figures rocprofv3 measured for this workload and these
+    kernel sources (profiles/traffic.json), or {} when none match."""
     try:
         with open(TRAFFIC_FILE) as f:
             table = json.load(f)
     except (OSError, ValueError):
-        return None, None, {}
+        return {}
     e = table.get(key)
     if not e or e.get("kernel_sources") != kernel_source_hash():
-        return None, None, {}
-    extra = {k: e[k] for k in ("valu_busy", "resident_waves_frac", "l2_hit") if k in e}
-    return e["read_bytes"] + e["write_bytes"], e.get("source"), extra
+        return {}
+    return e
 
 
-def roofline_of(kernel, alg_bytes, ms, bpv, key):
-    """The roofline object: algorithmic bytes (8 texel reads per sample at
-    the stored format, plus the images) over the launch time against HBM
-    peak, and, where a profile of these sources exists, the HBM bytes it
-    measured and the kernel's issue figures.  Texels are re-read from L1/L2
-    (neighbouring samples share taps), so the algorithmic rate can exceed
-    HBM peak; the measured rate shows what HBM actually carried."""
-    achieved = alg_bytes / (ms * 1e-3)
-    traffic, src, prof = measured_traffic(key)
-    out = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-           "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_source": src, "traffic_key": key,
-           "alg_bytes_per_launch": alg_bytes, "bytes_per_sample": bpv, "kernel": kernel, "avg_kernel_ms": ms}
-    if traffic is not None:
-        hbm = traffic / (ms * 1e-3)
-        out["hbm_measured_GBps"] = hbm / 1e9
-        out["hbm_measured_frac"] = hbm / HBM_PEAK
-        out["profile"] = prof
+def roofline_of(kernel, samples, bpv, out_bytes, ms, key):
+    """Roofline object of one launch of ``kernel``: measured HBM bytes per
+    launch over its duration against HBM peak (``frac``), the algorithmic
+    bytes beside it (``alg``), and the VALU issue roof (``valu``) from the
+    SQ counter pass."""
+    t = ms * 1e-3
+    alg_bytes = samples * bpv + out_bytes
+    prof = measured_profile(key)
+    traffic = None
+    if "read_bytes" in prof:
+        traffic = prof["read_bytes"] + prof["write_bytes"]
+    achieved = traffic / t if traffic is not None else None
+    out = {
+        "bound": "hbm", "achieved": None if achieved is None else achieved / 1e9, "peak": HBM_PEAK / 1e9,
+        "unit": "GB/s", "frac": None if achieved is None else achieved / HBM_PEAK, "traffic": traffic,
+        "kernel": kernel, "avg_kernel_ms": ms, "profile_key": key, "profile_source": prof.get("source"),
+        "alg": {"bytes_per_sample": bpv, "bytes_per_launch": alg_bytes, "achieved": alg_bytes / t / 1e9,
+                "frac": alg_bytes / t / HBM_PEAK},
+        "limiter": None,
+    }
+    if "valu_per_sample" in prof:
+        va = prof["valu_per_sample"] * samples / t
+        out["valu"] = {"instr_per_sample": prof["valu_per_sample"], "achieved": va / 1e12, "peak": VALU_PEAK / 1e12,
+                       "unit": "T lane-instructions/s", "frac": va / VALU_PEAK,
+                       "busy_counter": prof.get("valu_busy")}
+        out["limiter"] = "valu" if va / VALU_PEAK > (achieved or 0) / HBM_PEAK else "hbm"
+    for k in ("l2_hit", "resident_waves_frac"):
+        if k in prof:
+            out[k] = prof[k]
     return out
 
 
-def cpu_baseline(sc, cams, args, sf, nf):
-    """Oracle FP32 port of the shader (reference loop to max_steps) on all
-    host cores: whole frames of the workload's cameras, in order, until
-    ~cpu_seconds have elapsed."""
+# ---------------------------------------------------------------------------
+# CPU baseline
+
+def cpu_baseline(wl: Workload, data, normals, seconds):
+    """The oracle's FP32 port of the shader (reference loop to max_steps) on
+    all host cores: whole frames of the workload's cameras, in order, until
+    ``seconds`` have elapsed."""
     import oracle
     import scenes
 
     oracle.build()
     cores = len(os.sched_getaffinity(0))
-    data = scenes.dequant_scalar(sc.data, sf)
-    nrm = scenes.dequant_normals(sc.normals, nf) if sc.normals is not None else data
-    lut = sc.lut()
+    from pyvr_amd.hip_renderer.batch import rgba_lut
+
+    lut = rgba_lut(wl.ctf, wl.otf)
     samples, frames, t0 = 0, 0, time.perf_counter()
-    while time.perf_counter() - t0 < args.cpu_seconds:
-        u = sc.uniforms(cams[frames % len(cams)])
-        out = oracle.render(u, data, nrm, lut, mode="f32", full_loop=True, nthreads=cores,
-                            want=("counts",))
+    while time.perf_counter() - t0 < seconds:
+        q = wl.requests()[frames % len(wl.cams)]
+        u = scenes.request_uniforms(q, wl.volume)
+        out = oracle.render(u, data, normals if normals is not None else data, lut, mode="f32", full_loop=True,
+                            nthreads=cores, want=("counts",))
         samples += int(out["counts"].sum())
         frames += 1
     dt = time.perf_counter() - t0
     log(f"cpu baseline: {frames} frames, {samples} samples in {dt:.1f}s on {cores} threads")
     return {
         "value": samples / dt, "unit": "samples/s", "cores": cores, "kind": "port",
-        "sample": (f"{frames} whole {sc.width}x{sc.height} frames (cameras 0..{frames - 1} of the "
-                   f"workload, cycling), FP32 shader arithmetic, reference loop to max_steps, "
-                   f"{samples} samples in {dt:.1f}s; Mesa llvmpipe is not available on the host "
-                   "(no EGL/OSMesa/X server), so this C port of the shader stands in for it"),
+        "sample": (f"{frames} whole {wl.width}x{wl.height} frames (cameras 0..{frames - 1} of the workload, "
+                   f"cycling) on the volume's stored values read back from the device, FP32 shader arithmetic, "
+                   f"reference loop to max_steps, {samples} samples in {dt:.1f}s; Mesa llvmpipe is not "
+                   "available on the host (no EGL/OSMesa/X server), so this C port of the shader stands in "
+                   "for it"),
     }
 
 
-def bench_batch(args):
-    """c3: one batch of ``--requests`` requests (FP16 volumes of 64^3,
-    128^3, 256^3 and 96x160x224 with SNORM8x3 normals, images of 256^2 to
-    1024^2, own cameras and transfer functions) rendered by one persistent
-    graph-captured launch per run, against the same requests rendered one
-    per launch.  A step is one run of the whole batch, images copied to
-    host memory.  Plain figures, same requests and resident volumes:
-    plain_kernel -- back-to-back single-request launches on one stream
-    (LUT upload, parameters, ray setup and march per request; one wait at
-    the end); plain_api -- what a renderer user gets per frame (the same
-    plus the synchronous copy of each image to the host)."""
-    import scenes
-    from pyvr_amd.hip_renderer import BatchRenderer, _lib
-    from pyvr_amd.hip_renderer.batch import request_params, rgba_lut
-    from pyvr_amd.hip_renderer.manager import _DeviceVolume
+# ---------------------------------------------------------------------------
+# legs
 
-    sf, nf = args.scalar_format, args.normal_format
-    t0 = time.perf_counter()
-    vols = scenes.c3_volumes()
-    reqs = scenes.c3_requests(args.requests)
-    log(f"c3: {len(vols)} volumes, {len(reqs)} requests generated in {time.perf_counter() - t0:.1f}s")
+def plain_leg(wl: Workload, args, sf, nf):
+    """One launch per frame (ray setup + march) into resident device
+    buffers, back to back on one stream; then end-to-end per-frame calls."""
+    from pyvr_amd.hip_renderer import _lib
+    from pyvr_amd.hip_renderer.plain import PlainRenderer
+
     L = _lib.lib()
-    tuning = None
-    if args.tile_shape == "auto":
-        from pyvr_amd.hip_renderer.autotune import tune
+    lb = args.brick_log2 if args.brick_log2 >= 0 else 3
+    pr = PlainRenderer(scalar_format=sf, normal_format=nf, brick_log2=lb)
+    ctx = pr.ctx
+    _lib.check(L.hvr_set_empty_skip(ctx, {"off": _lib.SKIP_OFF, "on": _lib.SKIP_ON, "auto": _lib.SKIP_AUTO}[args.skip]),
+               "set_empty_skip")
+    t0 = time.perf_counter()
+    pr.upload(wl.volume)
+    vol_bytes = pr.resident_bytes()
+    log(f"volume resident: {vol_bytes / 2**20:.0f} MiB in {time.perf_counter() - t0:.1f}s")
+    params, counts = [], []
+    for cam in wl.cams:
+        pr.prepare(wl.volume, cam, wl.ctf, wl.otf, wl.width, wl.height, wl.config, wl.light, wl.tau)
+        params.append(pr.params)
+        counts.append(int(pr.sample_counts().sum()))
+    s_step = sum(counts)
+    npix = wl.width * wl.height
+    log(f"samples per step: {s_step} ({len(wl.cams)} frames)")
+    bufs = []
+    for _ in wl.cams:
+        p = ctypes.c_void_p()
+        _lib.check(L.hvr_device_alloc(ctx, npix * 4, ctypes.byref(p)), "alloc")
+        bufs.append(p)
 
-        t0 = time.perf_counter()
-        bricks = (args.brick_log2,) if args.brick_log2 >= 0 else (3, 4)
-        tuning = tune(vols, reqs, sf, nf, bricks=bricks)
-        lb, tile = tuning["brick_log2"], tuning["tile_shape"]
-        log(f"autotuner ({time.perf_counter() - t0:.1f}s): " +
-            ", ".join(f"lb{a}/{t} {ms:.2f} ms" for a, t, ms in tuning["table"]) + f" -> lb{lb}/{tile}")
-    else:
-        lb, tile = (args.brick_log2 if args.brick_log2 >= 0 else 3), args.tile_shape
+    def step():
+        for p, b in zip(params, bufs):
+            _lib.check(L.hvr_set_params(ctx, ctypes.byref(p)), "set_params")
+            _lib.check(L.hvr_render_async_to(ctx, b), "render")
 
-    b = BatchRenderer(scalar_format=sf, normal_format=nf, brick_log2=lb, tile_shape=tile, pipeline=2)
-    for q in reqs:
-        b.add(vols[q.volume], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
-    b.run(count=True)
-    samples = sum(int(b.counts(i).sum()) for i in range(len(reqs)))
-    pixels = sum(q.width * q.height for q in reqs)
-    log(f"batch: {samples} samples, {pixels} pixels per run; resident {b.resident_bytes() / 2**20:.0f} MiB")
-    # one batch at a time: launch, wait for the images in host memory
+    if args.copy_calibration_mib > 0:
+        nb = args.copy_calibration_mib << 20
+        a, b = ctypes.c_void_p(), ctypes.c_void_p()
+        _lib.check(L.hvr_device_alloc(ctx, nb, ctypes.byref(a)), "alloc")
+        _lib.check(L.hvr_device_alloc(ctx, nb, ctypes.byref(b)), "alloc")
+        _lib.check(L.hvr_device_copy(ctx, b, a, nb), "copy")
+        L.hvr_device_free(ctx, a)
+        L.hvr_device_free(ctx, b)
+        log(f"calibration copy: {args.copy_calibration_mib} MiB device-to-device")
+
     for _ in range(args.warmup):
-        b.run()
-    dev_ms = []
+        step()
+    _lib.check(L.hvr_synchronize(ctx), "sync")
+    _lib.check(L.hvr_timing_begin(ctx, args.steps * len(wl.cams)), "timing")
     t_start = time.perf_counter()
     for _ in range(args.steps):
-        b.run()
-        dev_ms.append(b.last_ms())
-    wall_sync = time.perf_counter() - t_start
-    value_sync = samples * args.steps / wall_sync
-    log(f"batched, one at a time: {wall_sync * 1e3 / args.steps:.2f} ms/run, {value_sync / 1e9:.2f} G samples/s "
-        f"(device {np.mean(dev_ms):.2f} ms/run)")
-    # two batches in flight (own streams): a run's copy back and tail
-    # overlap the next run's work; every run still ends with its images in
-    # host memory
-    s0 = b.submit()
-    for q in reqs:
-        b.add(vols[q.volume], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
-    s1 = b.submit()
-    b.wait(s0)
-    b.wait(s1)
-    for k in range(args.warmup):
-        b.rerun((s0, s1)[k % 2])
-    b.wait(s0)
-    b.wait(s1)
-    t_start = time.perf_counter()
-    for k in range(args.steps):
-        b.rerun((s0, s1)[k % 2])
-    b.wait(s0)
-    b.wait(s1)
+        step()
+    _lib.check(L.hvr_synchronize(ctx), "sync")
     wall = time.perf_counter() - t_start
-    value = samples * args.steps / wall
-    log(f"batched, two in flight: {wall * 1e3 / args.steps:.2f} ms/run, {value / 1e9:.2f} G samples/s")
+    march_ms, setup_ms, nfr = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+    _lib.check(L.hvr_timing_end(ctx, ctypes.byref(march_ms), ctypes.byref(setup_ms), ctypes.byref(nfr)),
+               "timing_end")
+    value = s_step * args.steps / wall
+    ms_step = wall * 1e3 / args.steps
+    log(f"plain: {args.steps} steps, {ms_step:.2f} ms/step, {value / 1e9:.2f} G samples/s; march kernel avg "
+        f"{march_ms.value / nfr.value:.3f} ms, setup {setup_ms.value / nfr.value:.3f} ms")
+    clear_b, total_b, used = ctypes.c_ulonglong(), ctypes.c_ulonglong(), ctypes.c_int()
+    _lib.check(L.hvr_skip_info(ctx, ctypes.byref(clear_b), ctypes.byref(total_b), ctypes.byref(used)), "skip_info")
+    skip = {"mode": args.skip, "used": bool(used.value), "clear_bricks": clear_b.value, "bricks": total_b.value}
+    for b in bufs:
+        L.hvr_device_free(ctx, b)
 
-    # plain: one context per image size, every volume resident in each
-    ctxs, dvols = {}, {}
-    for q in reqs:
-        key = (q.width, q.height)
-        if key in ctxs:
-            continue
-        c = ctypes.c_void_p()
-        _lib.check(L.hvr_create(0, q.width, q.height, ctypes.byref(c)), "hvr_create")
-        _lib.check(L.hvr_set_brick_log2(c, lb), "set_brick")
-        ctxs[key] = c
-        for k, v in enumerate(vols):
-            dv = _DeviceVolume(L, c, v.data.shape, sf, nf if v.has_normals else "none")
-            dv.upload_scalar(v.data)
-            if v.has_normals:
-                dv.upload_normals(v.normals)
-            dvols[key, k] = dv
-    bufs = {}
-    for key, c in ctxs.items():
-        p = ctypes.c_void_p()
-        _lib.check(L.hvr_device_alloc(c, key[0] * key[1] * 4, ctypes.byref(p)), "alloc")
-        bufs[key] = p
-    plan = []
-    for q in reqs:
-        key = (q.width, q.height)
-        lut = rgba_lut(q.ctf, q.otf)
-        prm = request_params(vols[q.volume], q.camera, q.config, q.light, q.width, q.height, q.tau)
-        plan.append((ctxs[key], dvols[key, q.volume].handle, lut, prm, bufs[key],
-                     np.empty(q.width * q.height * 4, np.uint8)))
-
-    def plain(read):
-        for c, vh, lut, prm, buf, host in plan:
-            nrole = _lib.HVR_ROLE_NORMAL
-            _lib.check(L.hvr_bind(c, vh, _lib.HVR_ROLE_SCALAR, vh, nrole), "bind")
-            _lib.check(L.hvr_set_lut(c, lut.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), lut.shape[0]), "lut")
-            _lib.check(L.hvr_set_params(c, ctypes.byref(prm)), "params")
-            if read:
-                _lib.check(L.hvr_render(c, host.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))), "render")
-            else:
-                _lib.check(L.hvr_render_async_to(c, buf), "render")
-        for c in ctxs.values():
-            _lib.check(L.hvr_synchronize(c), "sync")
-
-    res = {}
-    for name, read in (("plain_kernel", False), ("plain_api", True)):
-        for _ in range(max(1, args.warmup)):
-            plain(read)
+    # end-to-end frames: camera update, draw, RGBA8 copy to the host
+    lat = []
+    for i in range(args.latency_frames + 2):
+        cam = wl.cams[i % len(wl.cams)]
         t = time.perf_counter()
-        for _ in range(args.steps):
-            plain(read)
-        dt = time.perf_counter() - t
-        res[name] = samples * args.steps / dt
-        log(f"{name}: {dt * 1e3 / args.steps:.2f} ms/run, {res[name] / 1e9:.2f} G samples/s")
-    for key, c in ctxs.items():
-        L.hvr_device_free(c, bufs[key])
-    for dv in dvols.values():
-        dv.destroy()
-    for c in ctxs.values():
-        L.hvr_destroy(c)
-    b.close()
-
-    line = {
-        "metric": METRIC, "value": value, "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"c3: {len(reqs)} batched requests (FP16 blob volumes 64^3-256^3 and "
-                               f"96x160x224, images 256^2-1024^2), one persistent graph launch per run",
-                   "requests": len(reqs), "pixels_per_step": pixels, "samples_per_step": samples,
-                   "scalar_format": sf, "normal_format": nf, "brick_log2": lb, "tile_shape": tile,
-                   "autotuned": tuning is not None},
-        "autotune": None if tuning is None else [{"brick_log2": a, "tile_shape": t, "device_ms": ms}
-                                                 for a, t, ms in tuning["table"]],
-        "batch_device_ms": float(np.mean(dev_ms)), "batched_one_at_a_time": value_sync,
-        "plain_kernel": res["plain_kernel"], "plain_api": res["plain_api"],
-        "batched_vs_plain_kernel": value / res["plain_kernel"],
-        "batched_vs_plain_api": value / res["plain_api"],
-    }
-    print(json.dumps(line), flush=True)
+        pr.render(wl.volume, cam, wl.ctf, wl.otf, wl.width, wl.height, wl.config, wl.light, wl.tau)
+        if i >= 2:
+            lat.append((time.perf_counter() - t) * 1e3)
+    lat = np.array(lat)
+    stored = pr.download(wl.volume) if not args.no_cpu_baseline else (None, None)
+    pr.close()
+    bpv = 8 * (SCALAR_BYTES[sf] + (NORMAL_BYTES[nf] if wl.has_normals else 0))
+    key = f"{wl.name}/{len(wl.cams)}cams/{sf}+{nf if wl.has_normals else 'none'}/plain/lb{lb}/skip{int(used.value)}"
+    roofline = roofline_of("hvr::k_march", s_step / len(wl.cams), bpv, 4 * npix, march_ms.value / nfr.value, key)
+    return {"value": value, "ms_per_step": ms_step, "samples": s_step, "march_kernel_ms": march_ms.value / nfr.value,
+            "fps": 1e3 / float(lat.mean()), "frame_ms_p50": float(np.percentile(lat, 50)),
+            "frame_ms_p99": float(np.percentile(lat, 99)), "frames_per_s_device": len(wl.cams) * args.steps / wall,
+            "roofline": roofline, "brick_skip": skip, "volume_bytes": vol_bytes, "bpv": bpv}, stored
 
 
-def batched_leg(sc, cams, args, sf, nf, s_step):
+def batched_leg(wl: Workload, args, sf, nf, s_step):
     """The workload's cameras as one batch of requests (BatchRenderer: one
     persistent graph-captured launch per run, brick size and ray tile from
     the autotuner), two runs in flight on their own streams; every run ends
-    with all its images in pinned host memory.  Returns the figures for the
-    JSON line."""
-    import scenes
+    with all its images in pinned host memory."""
     from pyvr_amd.hip_renderer import BatchRenderer
 
-    vols = [sc.volume()]
-    reqs = [scenes.Request(0, cam, sc.ctf, sc.otf, sc.width, sc.height, sc.config, sc.light, sc.tau)
-            for cam in cams]
+    reqs = wl.requests()
+    vols = [wl.volume]
     tuning = None
     if args.tile_shape == "auto":
         from pyvr_amd.hip_renderer.autotune import tune
@@ -348,7 +337,6 @@ def batched_leg(sc, cams, args, sf, nf, s_step):
         samples = sum(int(b.counts(i).sum()) for i in range(len(reqs)))
         if samples != s_step:
             log(f"warning: batched sample count {samples} differs from the single-frame count {s_step}")
-        # one run at a time: latency of a whole batch (launch to images in host memory)
         for _ in range(args.warmup):
             b.run()
         lat, dev_ms = [], []
@@ -357,7 +345,6 @@ def batched_leg(sc, cams, args, sf, nf, s_step):
             b.run()
             lat.append((time.perf_counter() - t) * 1e3)
             dev_ms.append(b.last_ms())
-        # two runs in flight: the timed region of the value
         s0 = b.submit()
         for q in reqs:
             b.add(vols[0], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
@@ -388,34 +375,36 @@ def batched_leg(sc, cams, args, sf, nf, s_step):
                                                      for a, t, ms in tuning["table"]]}
 
 
-def bench_serve(args):
-    """c4: a serving loop over 16 resident FP16+SNORM8x3 blob volumes of
-    256^3 to 1024^3 (generated on the device), requests with Zipf(1.1)
-    volume popularity, square images of 512^2 to 2048^2 and the fast /
-    balanced / high_quality presets, arriving as a Poisson process at each
-    rate.  Latency runs from a request's (scheduled) arrival to its pixels
-    being in pinned host memory; throughput is the requests' samples over
-    the time from the first arrival to the last completion.  Samples come
-    from a counting pass over the same requests before the timed region."""
+def serving_leg(args, rates, duration, nvol, plain_requests):
+    """Serving: ``nvol`` resident FP16 blob volumes of 256^3 to 1024^3
+    (generated on the device), requests with Zipf(1.1) volume popularity,
+    square images of 512^2 to 2048^2 and the fast / balanced / high_quality
+    presets, arriving as a Poisson process at each rate.  Latency runs from
+    a request's (scheduled) arrival to its pixels being in pinned host
+    memory; throughput is the requests' samples over the time from the first
+    arrival to the last completion.  Samples come from a counting pass over
+    the same requests before the timed region.  Plain: the same requests one
+    per launch through the single-frame kernels, synchronously (each image
+    read back before the next request), on the same volumes."""
     import scenes
+    from pyvr_amd.hip_renderer.plain import PlainRenderer
     from pyvr_amd.hip_renderer.serve import RenderServer, ServeRequest
 
-    sf, nf = args.scalar_format, args.normal_format
+    sf, nf = args.serve_scalar_format, args.serve_normal_format
     lb = args.brick_log2 if args.brick_log2 >= 0 else 3
     tile = args.tile_shape if args.tile_shape != "auto" else "8x8"
-    rates = [float(r) for r in args.rates.split(",")]
-    n_max = max(50, int(max(rates) * args.duration))
-    vols = scenes.c4_volumes(args.volumes)
-    reqs = scenes.c4_requests(n_max, nvol=args.volumes)
+    n_max = max(50, int(max(rates) * duration))
+    vols = scenes.c4_volumes(nvol)
+    reqs = scenes.c4_requests(n_max, nvol=nvol)
     srv = RenderServer(scalar_format=sf, normal_format=nf, brick_log2=lb, tile_shape=tile,
-                       max_batch_requests=64, max_batch_pixels=1 << 26, pipeline=3, keep_results=False)
+                       max_batch_requests=args.serve_max_requests, max_batch_pixels=1 << 26, pipeline=3,
+                       keep_results=False, max_batch_samples=args.serve_max_samples)
     t0 = time.perf_counter()
     for k, v in enumerate(vols):
         srv.register(k, v)
     srv.preload()
-    log(f"c4: {len(srv.resident_keys())} volumes resident, {srv.resident_bytes() / 2**30:.1f} GiB, "
+    log(f"serving: {len(srv.resident_keys())} volumes resident, {srv.resident_bytes() / 2**30:.1f} GiB, "
         f"built in {time.perf_counter() - t0:.1f}s")
-    # counting pass (same kernels with sample counters), in chunks
     b = srv.backend
     samples = np.zeros(len(reqs), np.int64)
     t0 = time.perf_counter()
@@ -427,7 +416,7 @@ def bench_serve(args):
         for i in range(len(chunk)):
             samples[c0 + i] = int(b.counts(i).sum())
         b.clear()
-    log(f"counted {len(reqs)} requests in {time.perf_counter() - t0:.1f}s: "
+    log(f"serving: counted {len(reqs)} requests in {time.perf_counter() - t0:.1f}s, "
         f"{samples.mean() / 1e6:.0f} M samples per request on average")
 
     def run_rate(rate, n):
@@ -437,6 +426,7 @@ def bench_serve(args):
                                 height=q.height, config=q.config, light=q.light, tau=q.tau, tag=i)
                    for i, q in enumerate(reqs[:n])]
         done = []
+        srv.batches = 0
         start = time.perf_counter()
         i = 0
         while len(done) < n:
@@ -454,50 +444,182 @@ def bench_serve(args):
                 "samples_per_s": tot / (end - start), "achieved_req_per_s": n / (end - start),
                 "batches": srv.batches}
 
-    # plain: one request per launch, synchronously (no queueing), on the
-    # same resident volumes and kernels
-    n_plain = min(len(reqs), 200)
-    plain_lat = []
-    for q in reqs[:n_plain]:
-        b.add(vols[q.volume], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
-        t = time.perf_counter()
-        b.run()
-        plain_lat.append(time.perf_counter() - t)
-        b.clear()
-    plain_lat = np.array(plain_lat)
-    plain = {"requests": n_plain, "p50_ms": float(np.percentile(plain_lat, 50) * 1e3),
-             "p99_ms": float(np.percentile(plain_lat, 99) * 1e3),
-             "samples_per_s": float(samples[:n_plain].sum() / plain_lat.sum())}
-    log(f"plain, one request per launch: p50 {plain['p50_ms']:.2f} ms, p99 {plain['p99_ms']:.2f} ms, "
-        f"{plain['samples_per_s'] / 1e9:.1f} G samples/s")
-
-    # warm-up (graph capture for the shapes in use)
-    run_rate(max(rates), min(64, n_max))
+    run_rate(max(rates), min(64, n_max))      # warm-up: graphs for the shapes in use
     table = []
     for rate in rates:
-        srv.batches = 0
-        n = max(50, int(rate * args.duration))
+        n = max(50, int(rate * duration))
         row = run_rate(rate, n)
         table.append(row)
-        log(f"rate {rate:.0f}/s: {n} requests, p50 {row['p50_ms']:.2f} ms, p99 {row['p99_ms']:.2f} ms, "
+        log(f"serving {rate:.0f}/s: {n} requests, p50 {row['p50_ms']:.2f} ms, p99 {row['p99_ms']:.2f} ms, "
             f"{row['samples_per_s'] / 1e9:.1f} G samples/s, {row['achieved_req_per_s']:.0f} req/s, "
             f"{row['batches']} batches")
     srv.close()
+
+    # plain: one request per launch, synchronously, same requests and volumes
+    pr = PlainRenderer(scalar_format=sf, normal_format=nf, brick_log2=lb)
+    n_plain = min(len(reqs), plain_requests)
+    lat = []
+    try:
+        for q in reqs[:min(8, n_plain)]:      # warm-up (volumes resident, kernels loaded)
+            pr.render(vols[q.volume], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
+        for q in reqs[:n_plain]:
+            t = time.perf_counter()
+            pr.render(vols[q.volume], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
+            lat.append(time.perf_counter() - t)
+    finally:
+        pr.close()
+    lat = np.array(lat)
+    plain = {"requests": n_plain, "p50_ms": float(np.percentile(lat, 50) * 1e3),
+             "p99_ms": float(np.percentile(lat, 99) * 1e3),
+             "samples_per_s": float(samples[:n_plain].sum() / lat.sum()),
+             "path": "PlainRenderer.render: single-frame kernels, one request per launch, image read back"}
+    log(f"serving plain: p50 {plain['p50_ms']:.2f} ms, p99 {plain['p99_ms']:.2f} ms, "
+        f"{plain['samples_per_s'] / 1e9:.1f} G samples/s")
     best = max(table, key=lambda r: r["samples_per_s"])
+    return {"volumes": nvol, "scalar_format": sf, "normal_format": nf, "brick_log2": lb, "tile_shape": tile,
+            "max_batch_samples": args.serve_max_samples, "mean_samples_per_request": float(samples.mean()),
+            "rates": table, "plain": plain, "served_vs_plain": best["samples_per_s"] / plain["samples_per_s"]}
+
+
+# ---------------------------------------------------------------------------
+# c3: one batch of 64 mixed requests
+
+def bench_batch(args):
+    """c3: one batch of ``--requests`` requests (FP16 volumes of 64^3,
+    128^3, 256^3 and 96x160x224 with SNORM8x3 normals, images of 256^2 to
+    1024^2, own cameras and transfer functions) rendered by one persistent
+    graph-captured launch per run, against the same requests one per launch
+    through the single-frame kernels (plain_kernel: launches back to back
+    into resident buffers; plain_api: each image read back before the next
+    request)."""
+    import scenes
+    from pyvr_amd.hip_renderer import BatchRenderer, _lib
+    from pyvr_amd.hip_renderer.plain import PlainRenderer
+
+    sf, nf = args.scalar_format, args.normal_format
+    t0 = time.perf_counter()
+    vols = scenes.c3_volumes()
+    reqs = scenes.c3_requests(args.requests)
+    log(f"c3: {len(vols)} volumes, {len(reqs)} requests generated in {time.perf_counter() - t0:.1f}s")
+    L = _lib.lib()
+    tuning = None
+    if args.tile_shape == "auto":
+        from pyvr_amd.hip_renderer.autotune import tune
+
+        t0 = time.perf_counter()
+        bricks = (args.brick_log2,) if args.brick_log2 >= 0 else (3, 4)
+        tuning = tune(vols, reqs, sf, nf, bricks=bricks)
+        lb, tile = tuning["brick_log2"], tuning["tile_shape"]
+        log(f"autotuner ({time.perf_counter() - t0:.1f}s): " +
+            ", ".join(f"lb{a}/{t} {ms:.2f} ms" for a, t, ms in tuning["table"]) + f" -> lb{lb}/{tile}")
+    else:
+        lb, tile = (args.brick_log2 if args.brick_log2 >= 0 else 3), args.tile_shape
+
+    b = BatchRenderer(scalar_format=sf, normal_format=nf, brick_log2=lb, tile_shape=tile, pipeline=2)
+    for q in reqs:
+        b.add(vols[q.volume], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
+    b.run(count=True)
+    samples = sum(int(b.counts(i).sum()) for i in range(len(reqs)))
+    pixels = sum(q.width * q.height for q in reqs)
+    log(f"batch: {samples} samples, {pixels} pixels per run; resident {b.resident_bytes() / 2**20:.0f} MiB")
+    for _ in range(args.warmup):
+        b.run()
+    dev_ms = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        b.run()
+        dev_ms.append(b.last_ms())
+    wall_sync = time.perf_counter() - t_start
+    value_sync = samples * args.steps / wall_sync
+    log(f"batched, one at a time: {wall_sync * 1e3 / args.steps:.2f} ms/run, {value_sync / 1e9:.2f} G samples/s "
+        f"(device {np.mean(dev_ms):.2f} ms/run)")
+    s0 = b.submit()
+    for q in reqs:
+        b.add(vols[q.volume], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
+    s1 = b.submit()
+    b.wait(s0)
+    b.wait(s1)
+    for k in range(args.warmup):
+        b.rerun((s0, s1)[k % 2])
+    b.wait(s0)
+    b.wait(s1)
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        b.rerun((s0, s1)[k % 2])
+    b.wait(s0)
+    b.wait(s1)
+    wall = time.perf_counter() - t_start
+    value = samples * args.steps / wall
+    log(f"batched, two in flight: {wall * 1e3 / args.steps:.2f} ms/run, {value / 1e9:.2f} G samples/s")
+    b.close()
+
+    pr = PlainRenderer(scalar_format=sf, normal_format=nf, brick_log2=lb)
+    bufs = {}
+    for q in reqs:
+        key = (q.width, q.height)
+        if key not in bufs:
+            p = ctypes.c_void_p()
+            _lib.check(L.hvr_device_alloc(pr.ctx, q.width * q.height * 4, ctypes.byref(p)), "alloc")
+            bufs[key] = p
+
+    def plain(read):
+        for q in reqs:
+            if read:
+                pr.render(vols[q.volume], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
+            else:
+                pr.prepare(vols[q.volume], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
+                pr.launch_to(bufs[q.width, q.height])
+        _lib.check(L.hvr_synchronize(pr.ctx), "sync")
+
+    res = {}
+    for name, read in (("plain_kernel", False), ("plain_api", True)):
+        for _ in range(max(1, args.warmup)):
+            plain(read)
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            plain(read)
+        dt = time.perf_counter() - t
+        res[name] = samples * args.steps / dt
+        log(f"{name}: {dt * 1e3 / args.steps:.2f} ms/run, {res[name] / 1e9:.2f} G samples/s")
+    for p in bufs.values():
+        L.hvr_device_free(pr.ctx, p)
+    pr.close()
+    line = {
+        "metric": METRIC, "value": value, "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"c3: {len(reqs)} batched requests (FP16 blob volumes 64^3-256^3 and "
+                               f"96x160x224, images 256^2-1024^2), one persistent graph launch per run",
+                   "requests": len(reqs), "pixels_per_step": pixels, "samples_per_step": samples,
+                   "scalar_format": sf, "normal_format": nf, "brick_log2": lb, "tile_shape": tile,
+                   "autotuned": tuning is not None},
+        "autotune": None if tuning is None else [{"brick_log2": a, "tile_shape": t, "device_ms": ms}
+                                                 for a, t, ms in tuning["table"]],
+        "batch_device_ms": float(np.mean(dev_ms)), "batched_one_at_a_time": value_sync,
+        "plain_kernel": res["plain_kernel"], "plain_api": res["plain_api"],
+        "batched_vs_plain_kernel": value / res["plain_kernel"],
+        "batched_vs_plain_api": value / res["plain_api"],
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_serve(args):
+    """c4 on its own: the serving leg over the full rate sweep."""
+    rates = [float(r) for r in args.rates.split(",")]
+    srv = serving_leg(args, rates, args.duration, args.volumes, 200)
+    best = max(srv["rates"], key=lambda r: r["samples_per_s"])
     line = {
         "metric": METRIC, "value": best["samples_per_s"], "unit": "samples/s", "n_gpus": 1,
         "steps": len(rates), "warmup": 1, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "c4: serving loop, 16 resident FP16+SNORM8x3 blob volumes 256^3-1024^3, "
-                               "Poisson arrivals, images 512^2-2048^2, fast/balanced/high_quality",
-                   "volumes": args.volumes, "requests_max": n_max, "scalar_format": sf, "normal_format": nf,
-                   "brick_log2": lb, "tile_shape": tile, "mean_samples_per_request": float(samples.mean())},
-        "serving": table,
-        "plain": plain,
-        "served_vs_plain": best["samples_per_s"] / plain["samples_per_s"],
+        "config": {"workload": f"c4: serving loop, {args.volumes} resident FP16 blob volumes 256^3-1024^3, "
+                               "Poisson arrivals, images 512^2-2048^2, fast/balanced/high_quality"},
+        "serving": srv,
     }
     print(json.dumps(line), flush=True)
 
+
+# ---------------------------------------------------------------------------
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
@@ -507,23 +629,31 @@ def main():
     ap.add_argument("--workload", default="c2", choices=["c0", "c1", "c2", "c3", "c4"])
     ap.add_argument("--rates", default="50,100,200,500,1000,2000", help="c4: Poisson arrival rates (req/s)")
     ap.add_argument("--duration", type=float, default=2.0, help="c4: seconds of arrivals per rate")
-    ap.add_argument("--volumes", type=int, default=16, help="c4: resident volumes")
+    ap.add_argument("--volumes", type=int, default=16, help="serving: resident volumes")
     ap.add_argument("--requests", type=int, default=64, help="c3: requests per batch")
     ap.add_argument("--tile-shape", default="auto", choices=["auto", "8x8", "16x4"],
-                    help="c3: ray tile of a wavefront (auto: the autotuner picks it and the brick size)")
+                    help="ray tile of a wavefront (auto: the autotuner picks it and the brick size)")
     ap.add_argument("--size", type=int, default=0)
     ap.add_argument("--image", type=int, default=0)
     ap.add_argument("--cameras", type=int, default=24)
     ap.add_argument("--scalar-format", default="f16", choices=["f32", "f16", "u8"])
-    ap.add_argument("--normal-format", default="sn8x3", choices=["f32x3", "sn8x3", "f16x3"])
+    ap.add_argument("--normal-format", default="f16x3", choices=["f32x3", "sn8x3", "f16x3"])
+    ap.add_argument("--serve-scalar-format", default="f16", choices=["f32", "f16", "u8"])
+    ap.add_argument("--serve-normal-format", default="sn8x3", choices=["f32x3", "sn8x3", "f16x3"])
+    ap.add_argument("--serve-rates", default="500,1000", help="serving leg of the default run (req/s)")
+    ap.add_argument("--serve-duration", type=float, default=1.0, help="serving leg: seconds of arrivals per rate")
+    ap.add_argument("--serve-max-requests", type=int, default=64)
+    ap.add_argument("--serve-max-samples", type=float, default=None,
+                    help="serving: bound a batch's estimated samples (None: requests and pixels only)")
+    ap.add_argument("--no-serve", action="store_true", help="default run without the serving leg")
     ap.add_argument("--brick-log2", type=int, default=-1)
     ap.add_argument("--skip", choices=("auto", "on", "off"), default="auto",
                     help="transparent-brick skipping mode (result is identical in every mode)")
     ap.add_argument("--latency-frames", type=int, default=48)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plain-only", action="store_true",
-                    help="c2: time only the one-frame-per-launch renderer (no batched leg)")
+                    help="c2: time only the one-frame-per-launch renderer (no batched or serving leg)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="before the timed region, copy this many MiB device-to-device once "
                          "(a known-size kernel for calibrating traffic counters)")
@@ -533,142 +663,50 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); this renderer's "
               "multi-GPU split is a design note, not a measured path.", file=sys.stderr)
         sys.exit(2)
-
     if args.workload == "c3":
         return bench_batch(args)
     if args.workload == "c4":
         return bench_serve(args)
 
-    from pyvr_amd.hip_renderer import VolumeRenderer, _lib
-
-    t0 = time.perf_counter()
-    sc, cams = make_workload(args)
     sf, nf = args.scalar_format, args.normal_format
-    log(f"workload {sc.name}: generated in {time.perf_counter() - t0:.1f}s")
-
-    r = VolumeRenderer(sc.width, sc.height, config=sc.config, light=sc.light,
-                       scalar_format=sf, normal_format=nf)
-    L = _lib.lib()
-    ctx = r.gl_manager.ctx
-    if args.brick_log2 >= 0:
-        _lib.check(L.hvr_set_brick_log2(ctx, args.brick_log2), "set_brick")
-    _lib.check(L.hvr_set_empty_skip(ctx, {"off": _lib.SKIP_OFF, "on": _lib.SKIP_ON,
-                                           "auto": _lib.SKIP_AUTO}[args.skip]), "set_empty_skip")
-    t0 = time.perf_counter()
-    r.load_volume(sc.volume())
-    vol_bytes = sum(v.nbytes() for v in r.gl_manager._volumes)
-    log(f"volume resident: {vol_bytes / 2**20:.0f} MiB in {time.perf_counter() - t0:.1f}s")
-    r.set_transfer_functions(sc.ctf, sc.otf, sc.lut_size)
-    r.set_opacity_threshold(sc.tau)
-
-    # per-camera parameter blocks, exactly as set_camera + render would set them
-    params, counts = [], []
-    for cam in cams:
-        r.set_camera(cam)
-        r.gl_manager.clear_framebuffer(0, 0, 0, 0)
-        r.gl_manager.setup_blending()
-        r.gl_manager._sync_bindings()
-        params.append(r.gl_manager._params())
-        counts.append(int(r.sample_counts().sum()))
-    s_step = sum(counts)
-    npix = sc.width * sc.height
-    log(f"samples per step: {s_step} ({len(cams)} frames)")
-
-    bufs = []
-    for _ in cams:
-        p = ctypes.c_void_p()
-        _lib.check(L.hvr_device_alloc(ctx, npix * 4, ctypes.byref(p)), "alloc")
-        bufs.append(p)
-
-    def step():
-        for p, b in zip(params, bufs):
-            _lib.check(L.hvr_set_params(ctx, ctypes.byref(p)), "set_params")
-            _lib.check(L.hvr_render_async_to(ctx, b), "render")
-
-    if args.copy_calibration_mib > 0:
-        nb = args.copy_calibration_mib << 20
-        a, b = ctypes.c_void_p(), ctypes.c_void_p()
-        _lib.check(L.hvr_device_alloc(ctx, nb, ctypes.byref(a)), "alloc")
-        _lib.check(L.hvr_device_alloc(ctx, nb, ctypes.byref(b)), "alloc")
-        _lib.check(L.hvr_device_copy(ctx, b, a, nb), "copy")
-        L.hvr_device_free(ctx, a)
-        L.hvr_device_free(ctx, b)
-        log(f"calibration copy: {args.copy_calibration_mib} MiB device-to-device")
-
-    for _ in range(args.warmup):
-        step()
-    _lib.check(L.hvr_synchronize(ctx), "sync")
-    _lib.check(L.hvr_timing_begin(ctx, args.steps * len(cams)), "timing")
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    _lib.check(L.hvr_synchronize(ctx), "sync")
-    wall = time.perf_counter() - t_start
-    march_ms, setup_ms, nfr = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
-    _lib.check(L.hvr_timing_end(ctx, ctypes.byref(march_ms), ctypes.byref(setup_ms), ctypes.byref(nfr)),
-               "timing_end")
-    value = s_step * args.steps / wall
-    ms_step = wall * 1e3 / args.steps
-    log(f"timed {args.steps} steps: {ms_step:.2f} ms/step, {value / 1e9:.2f} G samples/s; "
-        f"march kernel avg {march_ms.value / nfr.value:.3f} ms, setup {setup_ms.value / nfr.value:.3f} ms")
-
-    clear_b, total_b, used = ctypes.c_ulonglong(), ctypes.c_ulonglong(), ctypes.c_int()
-    _lib.check(L.hvr_skip_info(ctx, ctypes.byref(clear_b), ctypes.byref(total_b), ctypes.byref(used)), "skip_info")
-    skip = {"mode": args.skip, "used": bool(used.value), "clear_bricks": clear_b.value, "bricks": total_b.value}
-    log(f"transparent bricks {clear_b.value}/{total_b.value}; stepped over: {bool(used.value)}")
-
-    bpv = 8 * (SCALAR_BYTES[sf] + (NORMAL_BYTES[nf] if sc.normals is not None else 0))
-    alg_bytes = s_step * args.steps * bpv + 4 * npix * nfr.value
-    tkey = (f"{sc.name}/{len(cams)}cams/{sf}+{nf if sc.normals is not None else 'none'}/lb{args.brick_log2}"
-            f"/skip{int(used.value)}")
-    roofline = roofline_of("hvr::k_march", alg_bytes / nfr.value, march_ms.value / nfr.value, bpv, tkey)
-
-    # end-to-end frames through the public API (set_camera + render + D2H)
-    lat = []
-    for i in range(args.latency_frames + 2):
-        cam = cams[i % len(cams)]
-        t = time.perf_counter()
-        r.set_camera(cam)
-        r.render()
-        if i >= 2:
-            lat.append((time.perf_counter() - t) * 1e3)
-    lat = np.array(lat)
-
-    for b in bufs:
-        L.hvr_device_free(ctx, b)
-
-    plain = {"value": value, "ms_per_step": ms_step, "march_kernel_ms": march_ms.value / nfr.value,
-             "fps": 1e3 / float(lat.mean()), "frame_ms_p50": float(np.percentile(lat, 50)),
-             "frame_ms_p99": float(np.percentile(lat, 99)),
-             "frames_per_s_device": len(cams) * args.steps / wall, "roofline": roofline}
-    r.close()
+    wl = make_workload(args)
+    log(f"workload {wl.name}: {len(wl.cams)} cameras, {wl.width}x{wl.height}, volume {wl.shape}")
+    plain, stored = plain_leg(wl, args, sf, nf)
+    s_step, bpv, npix = plain["samples"], plain["bpv"], wl.width * wl.height
 
     batched = None
+    roofline = plain["roofline"]
     if args.workload == "c2" and not args.plain_only:
-        batched = batched_leg(sc, cams, args, sf, nf, s_step)
-        alg_b = batched["samples"] * bpv + 4 * npix * len(cams)
-        bkey = f"{sc.name}/{len(cams)}cams/{sf}+{nf}/batch/lb{batched['brick_log2']}/{batched['tile_shape']}"
+        batched = batched_leg(wl, args, sf, nf, s_step)
+        bkey = f"{wl.name}/{len(wl.cams)}cams/{sf}+{nf}/batch/lb{batched['brick_log2']}/{batched['tile_shape']}"
         roofline = roofline_of("hvr::k_batch (graph launch: staging copy, counter fill, brick flags, k_batch)",
-                               alg_b, batched["device_ms_per_run"], bpv, bkey)
-
+                               batched["samples"], bpv, 4 * npix * len(wl.cams), batched["device_ms_per_run"], bkey)
+    serving = None
+    if args.workload == "c2" and not args.plain_only and not args.no_serve:
+        serving = serving_leg(args, [float(r) for r in args.serve_rates.split(",")], args.serve_duration,
+                              args.volumes, 100)
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(sc, cams, args, sf, nf)
+        cpu = cpu_baseline(wl, stored[0], stored[1], args.cpu_seconds)
 
+    plain_out = {k: plain[k] for k in ("value", "ms_per_step", "march_kernel_ms", "fps", "frame_ms_p50",
+                                       "frame_ms_p99", "frames_per_s_device", "roofline")}
     if batched is not None:
         extra = {
             "fps": batched["frames_per_s"],
             "batch_ms_p50": batched["batch_ms_p50"], "batch_ms_p99": batched["batch_ms_p99"],
             "frame_ms_p50": plain["frame_ms_p50"], "frame_ms_p99": plain["frame_ms_p99"],
             "batched": {k: batched[k] for k in ("device_ms_per_run", "brick_log2", "tile_shape", "autotune")},
-            "plain_kernel": plain,
-            "batched_vs_plain_kernel": batched["value"] / value,
+            "plain_kernel": plain_out,
+            "batched_vs_plain_kernel": batched["value"] / plain["value"],
         }
         value, ms_step = batched["value"], batched["ms_per_step"]
     else:
-        extra = {"fps": plain["fps"], "frame_ms_p50": plain["frame_ms_p50"],
-                 "frame_ms_p99": plain["frame_ms_p99"], "frames_per_s_device": plain["frames_per_s_device"]}
-
+        extra = {"fps": plain["fps"], "frame_ms_p50": plain["frame_ms_p50"], "frame_ms_p99": plain["frame_ms_p99"],
+                 "frames_per_s_device": plain["frames_per_s_device"]}
+        value, ms_step = plain["value"], plain["ms_per_step"]
+    if serving is not None:
+        extra["serving"] = serving
     line = {
         "metric": METRIC,
         "value": value,
@@ -683,16 +721,14 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": (f"{sc.name}: {len(cams)} cameras, {sc.width}x{sc.height}, "
-                         f"{sc.data.shape[0]}^3 blob volume" if args.workload != "c0" else sc.name)
-                        + ("; the cameras batched into one persistent graph launch per step"
-                           if batched is not None else "; one launch per frame"),
-            "volume": list(sc.data.shape), "image": [sc.width, sc.height],
-            "cameras": len(cams), "preset": {"step_size": sc.config.step_size,
-                                             "max_steps": sc.config.max_steps},
-            "et_threshold": sc.tau, "scalar_format": sf,
-            "normal_format": nf if sc.normals is not None else "none",
-            "samples_per_step": s_step, "volume_bytes": vol_bytes, "brick_skip": skip,
+            "workload": (f"{wl.name}: {len(wl.cams)} cameras, {wl.width}x{wl.height}, "
+                         f"{'x'.join(map(str, wl.shape))} volume"
+                         + ("; the cameras batched into one persistent graph launch per step"
+                            if batched is not None else "; one launch per frame")),
+            "volume": list(wl.shape), "image": [wl.width, wl.height],
+            "cameras": len(wl.cams), "preset": {"step_size": wl.config.step_size, "max_steps": wl.config.max_steps},
+            "et_threshold": wl.tau, "scalar_format": sf, "normal_format": nf if wl.has_normals else "none",
+            "samples_per_step": s_step, "volume_bytes": plain["volume_bytes"], "brick_skip": plain["brick_skip"],
         },
         **extra,
         "roofline": roofline,

@@ -90,6 +90,20 @@ typedef struct {
 /* Version of this ABI (HVR_ABI_VERSION). */
 int hvr_abi_version(void);
 
+/* Debugging aid (no counterpart in the reference): the entry point named
+ * `function` (e.g. "hvr_batch_run_async") fails with `code` (a negative
+ * HVR_E* value) on its `after`-th call from now (0: the next call), once,
+ * before doing any device work, with the message "injected fault in
+ * <function>".  NULL disarms.  For testing how callers handle failures
+ * without a failing device.  Process-wide; one armed entry point at a time.
+ * Entry points with an injection point: hvr_create, hvr_resize,
+ * hvr_volume_create, hvr_volume_upload_scalar, hvr_volume_upload_normals,
+ * hvr_volume_upload_scalar_normals, hvr_volume_generate_blobs, hvr_set_lut,
+ * hvr_set_params, hvr_bind, hvr_render_async, hvr_render_async_to,
+ * hvr_read, hvr_sample_counts, hvr_batch_create, hvr_batch_add,
+ * hvr_batch_run_async, hvr_batch_wait, hvr_batch_query. */
+int hvr_debug_inject_fault(const char *function, int code, int after);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char *hvr_last_error(void);
 
@@ -108,6 +122,13 @@ int hvr_device_memory(int device, unsigned long long *free_bytes, unsigned long 
  * (same results; for testing that path). */
 int hvr_create(int device, int width, int height, hvr_ctx **out);
 int hvr_destroy(hvr_ctx *ctx);
+/* Change the framebuffer size (no counterpart in the reference, whose
+ * renderer keeps the size it was created with: a new size there means a new
+ * ModernGLManager, manager.py:14-41).  Volumes, LUT and bindings stay; the
+ * parameters' matrices are the caller's (aspect = width / height); buffers
+ * grow when the new size needs more pixels and are kept otherwise.
+ * Waits for the context's stream. */
+int hvr_resize(hvr_ctx *ctx, int width, int height);
 
 /* ---- volumes: replace create_volume_texture (manager.py:77-108) and
  * create_normal_texture (manager.py:110-135) ----
@@ -144,6 +165,13 @@ int hvr_volume_generate_blobs(hvr_volume *vol, int n_blobs, const double *center
 
 /* Device bytes held by the volume's texels. */
 int hvr_volume_bytes(const hvr_volume *vol, uint64_t *bytes);
+/* The stored fields read back from the device, in numpy C order of the
+ * volume's shape (no counterpart in the reference, whose textures are never
+ * read back): `scalar` s0*s1*s2 float32 as the march reads them (FP16
+ * widened, UNORM8 times 1/255), `normals` s0*s1*s2*3 float32 as stored
+ * (SNORM8 as its integer in [-127, 127], FP16 widened); either may be NULL.
+ * For checking device-built volumes against a CPU renderer.  Waits. */
+int hvr_volume_download(hvr_volume *vol, float *scalar, float *normals);
 int hvr_volume_destroy(hvr_volume *vol);
 
 /* ---- transfer function: replaces create_rgba_transfer_function_texture

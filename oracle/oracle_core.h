@@ -61,8 +61,11 @@ static void SUF(tex3)(const ovr_tex3 *t, REAL cx, REAL cy, REAL cz, REAL out[3])
 static void SUF(lut1)(const float *lut, int n, REAL rho, REAL out[4])
 {
     REAL u = rho * (REAL)n - (REAL)0.5;
-    if (u < (REAL)-1) u = (REAL)-1;           /* keeps the int conversion in range; */
-    if (u > (REAL)n) u = (REAL)n;             /* both clamps pick the same edge texel */
+    /* keep the int conversion in range (both clamps pick the same edge
+     * texel); a NaN density reads the first entry, as max(NaN, 0) = 0 in
+     * the IEEE maxNum clamp */
+    if (!(u >= (REAL)-1)) u = (REAL)-1;
+    if (u > (REAL)n) u = (REAL)n;
     REAL fu = FLOOR_FN(u);
     REAL a = u - fu;
     long i0 = (long)fu, i1 = i0 + 1;

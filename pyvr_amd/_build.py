@@ -10,7 +10,7 @@ import subprocess
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
 SOURCES = [os.path.join(_PKG, "csrc", "hipvr.hip")]
-DEPENDS = SOURCES + [os.path.join(_PKG, "csrc", "hvr_device.h"),
+DEPENDS = SOURCES + [os.path.join(_PKG, "csrc", "hvr_device.h"), os.path.join(_PKG, "csrc", "hvr_rules.h"),
                      os.path.join(os.path.dirname(_PKG), "include", "hipvr.h")]
 OUTPUT = os.path.join(_PKG, "libhipvr.so")
 ARCH = os.environ.get("PYTORCH_ROCM_ARCH", "gfx950").split(";")[0] or "gfx950"

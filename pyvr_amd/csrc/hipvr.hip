@@ -12,6 +12,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -35,6 +37,29 @@ static int fail_hip(hipError_t e, const char *where)
     do {                                               \
         hipError_t e_ = (expr);                        \
         if (e_ != hipSuccess) return fail_hip(e_, #expr); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// fault injection (hvr_debug_inject_fault): one armed entry point fails with
+// a chosen code on its n-th call, before any device work
+static std::mutex g_fault_mu;
+static std::atomic<bool> g_fault_armed{false};
+static std::string g_fault_fn;
+static int g_fault_code = 0, g_fault_after = -1;
+
+static int fault_point(const char *fn)
+{
+    if (!g_fault_armed.load(std::memory_order_relaxed)) return HVR_OK;
+    std::lock_guard<std::mutex> lock(g_fault_mu);
+    if (g_fault_after < 0 || g_fault_fn != fn) return HVR_OK;
+    if (g_fault_after-- > 0) return HVR_OK;
+    g_fault_armed.store(false);
+    g_fault_after = -1;
+    return fail(g_fault_code, (std::string("injected fault in ") + fn).c_str());
+}
+#define HVR_FAULT_POINT(fn)                             \
+    do {                                               \
+        if (const int rc_ = fault_point(fn)) return rc_; \
     } while (0)
 
 // ---------------------------------------------------------------------------
@@ -68,6 +93,7 @@ struct hvr_ctx {
     size_t stage_bytes;
     hipEvent_t ev[4];
     hipEvent_t rev[4];   // readback chunks (hvr_read)
+    hipEvent_t last_ev[3];  // the last frame's events (ev[] or a timing region's)
     bool events_recorded;
     hvr_params params;
     bool params_set;
@@ -89,6 +115,9 @@ struct hvr_ctx {
     int skip_mode;         // HVR_SKIP_OFF / _ON / _AUTO
     int skip_used;         // whether the last frame's march stepped over bricks
     bool force_global;     // HVR_GATHER=global at creation: 64-bit loads for every volume
+    int ncu;               // compute units of the device
+    size_t pix_cap;        // pixels d_out, d_rays and h_pinned hold
+    size_t counts_cap;     // pixels d_counts holds
 };
 
 // HVR_SKIP_AUTO steps over transparent bricks when at least this fraction of
@@ -98,38 +127,6 @@ static constexpr double kSkipAutoFraction = 0.25;
 
 // ---------------------------------------------------------------------------
 // FP64 uniform preparation
-
-// Inverse of a 4x4 matrix (row-major, m[4*r + c]) by Gauss-Jordan elimination
-// with partial pivoting.  Returns false when singular.
-static bool inverse4(const double *m, double *inv)
-{
-    double a[4][8];
-    for (int r = 0; r < 4; ++r)
-        for (int c = 0; c < 8; ++c) a[r][c] = c < 4 ? m[4 * r + c] : (c - 4 == r ? 1.0 : 0.0);
-    for (int col = 0; col < 4; ++col) {
-        int piv = col;
-        for (int r = col + 1; r < 4; ++r)
-            if (fabs(a[r][col]) > fabs(a[piv][col])) piv = r;
-        if (a[piv][col] == 0.0) return false;
-        if (piv != col)
-            for (int c = 0; c < 8; ++c) {
-                double t = a[col][c];
-                a[col][c] = a[piv][c];
-                a[piv][c] = t;
-            }
-        const double d = a[col][col];
-        for (int c = 0; c < 8; ++c) a[col][c] /= d;
-        for (int r = 0; r < 4; ++r) {
-            if (r == col) continue;
-            const double f = a[r][col];
-            if (f != 0.0)
-                for (int c = 0; c < 8; ++c) a[r][c] -= f * a[col][c];
-        }
-    }
-    for (int r = 0; r < 4; ++r)
-        for (int c = 0; c < 4; ++c) inv[4 * r + c] = a[r][c + 4];
-    return true;
-}
 
 // Uniforms -> kernel constants.  The matrices arrive as numpy row-major
 // float32 bytes that GLSL reads column-major, i.e. transposed
@@ -211,11 +208,11 @@ static TexDesc tex_desc(const hvr_volume *v, int role, const FrameConst &fc)
 // kernel dispatch
 template <int SF, int NM, int LB, bool COUNT, bool SKIP = false, bool BUF = false>
 static void launch_march_t(hvr_ctx *c, const FrameConst &fc, const TexDesc &ts, const TexDesc &tn,
-                           int ntx, int nblocks, const uint8_t *skip)
+                           int ntx, int nblocks, const uint8_t *skip, uchar4 *out)
 {
     const size_t lds = NM == NM_ANY ? 0 : 2 * (size_t)c->lut_n * sizeof(float4);
     hipLaunchKernelGGL((k_march<SF, NM, LB, COUNT, SKIP, BUF>), dim3(nblocks), dim3(256), lds, c->stream, fc,
-                       ts, tn, (const float4 *)c->d_lut, c->lut_n, (const RayRec *)c->d_rays, c->d_out,
+                       ts, tn, (const float4 *)c->d_lut, c->lut_n, (const RayRec *)c->d_rays, out,
                        c->d_counts, skip, ntx, nblocks);
 }
 
@@ -261,32 +258,34 @@ static const uint8_t *brick_flags(hvr_ctx *c, const hvr_volume *v)
 
 template <int SF, int LB, bool COUNT, bool SKIP, bool BUF>
 static void launch_fast_nm(hvr_ctx *c, const FrameConst &fc, const TexDesc &ts, int nm, int ntx, int nblocks,
-                           const uint8_t *skip)
+                           const uint8_t *skip, uchar4 *out)
 {
     const TexDesc none = tex_desc(nullptr, 0, fc);
-    if (nm == NM_ZERO) launch_march_t<SF, NM_ZERO, LB, COUNT, SKIP, BUF>(c, fc, ts, none, ntx, nblocks, skip);
-    else if (nm == NM_ALIAS) launch_march_t<SF, NM_ALIAS, LB, COUNT, SKIP, BUF>(c, fc, ts, none, ntx, nblocks, skip);
+    if (nm == NM_ZERO) launch_march_t<SF, NM_ZERO, LB, COUNT, SKIP, BUF>(c, fc, ts, none, ntx, nblocks, skip, out);
+    else if (nm == NM_ALIAS)
+        launch_march_t<SF, NM_ALIAS, LB, COUNT, SKIP, BUF>(c, fc, ts, none, ntx, nblocks, skip, out);
     else if constexpr (fmt_has_normal(SF))
-        launch_march_t<SF, NM_SAME, LB, COUNT, SKIP, BUF>(c, fc, ts, none, ntx, nblocks, skip);
+        launch_march_t<SF, NM_SAME, LB, COUNT, SKIP, BUF>(c, fc, ts, none, ntx, nblocks, skip, out);
 }
 
 // Fast-path instance: transparent-brick skipping when the flags say so,
 // buffer loads when the volume's allocation is below 4 GiB.
 template <int SF, int LB, bool COUNT>
 static void launch_fast(hvr_ctx *c, const FrameConst &fc, const TexDesc &ts, const hvr_volume *S, int nm,
-                        int ntx, int nblocks)
+                        int ntx, int nblocks, uchar4 *out)
 {
     const uint8_t *skip = brick_flags(c, S);
     c->skip_used = skip != nullptr;
     const bool buf = ts.nbytes32 != 0 && !c->force_global;
-    if (skip && buf) launch_fast_nm<SF, LB, COUNT, true, true>(c, fc, ts, nm, ntx, nblocks, skip);
-    else if (skip) launch_fast_nm<SF, LB, COUNT, true, false>(c, fc, ts, nm, ntx, nblocks, skip);
-    else if (buf) launch_fast_nm<SF, LB, COUNT, false, true>(c, fc, ts, nm, ntx, nblocks, nullptr);
-    else launch_fast_nm<SF, LB, COUNT, false, false>(c, fc, ts, nm, ntx, nblocks, nullptr);
+    if (skip && buf) launch_fast_nm<SF, LB, COUNT, true, true>(c, fc, ts, nm, ntx, nblocks, skip, out);
+    else if (skip) launch_fast_nm<SF, LB, COUNT, true, false>(c, fc, ts, nm, ntx, nblocks, skip, out);
+    else if (buf) launch_fast_nm<SF, LB, COUNT, false, true>(c, fc, ts, nm, ntx, nblocks, nullptr, out);
+    else launch_fast_nm<SF, LB, COUNT, false, false>(c, fc, ts, nm, ntx, nblocks, nullptr, out);
 }
 
+// The march of one frame into `out` (W*H RGBA8 on the device).
 template <bool COUNT>
-static void launch_march(hvr_ctx *c, const FrameConst &fc)
+static void launch_march(hvr_ctx *c, const FrameConst &fc, uchar4 *out)
 {
     const int ntx = (c->W + 15) / 16, nty = (c->H + 15) / 16;
     const int nblocks = ntx * nty;
@@ -296,7 +295,7 @@ static void launch_march(hvr_ctx *c, const FrameConst &fc)
     const bool small_lut = c->lut_n <= kLutLdsMax;
     c->skip_used = 0;
     if (small_lut && !S && !N) {
-        launch_march_t<TF_NONE, NM_ZERO, 3, COUNT>(c, fc, none, none, ntx, nblocks, nullptr);
+        launch_march_t<TF_NONE, NM_ZERO, 3, COUNT>(c, fc, none, none, ntx, nblocks, nullptr, out);
         return;
     }
     // Fast paths: scalar field of one volume, normals from the same texels,
@@ -308,8 +307,8 @@ static void launch_march(hvr_ctx *c, const FrameConst &fc)
         else if (N == S && c->normal_role == HVR_ROLE_NORMAL && fmt_has_scalar(S->fmt)) nm = NM_SAME;
 #define HVR_CASE(SF)                                                                          \
     case SF:                                                                                  \
-        if (S->lb == 3) launch_fast<SF, 3, COUNT>(c, fc, ts, S, nm, ntx, nblocks);            \
-        else launch_fast<SF, 4, COUNT>(c, fc, ts, S, nm, ntx, nblocks);                       \
+        if (S->lb == 3) launch_fast<SF, 3, COUNT>(c, fc, ts, S, nm, ntx, nblocks, out);       \
+        else launch_fast<SF, 4, COUNT>(c, fc, ts, S, nm, ntx, nblocks, out);                  \
         return;
         if (nm >= 0) {
             switch (S->fmt) {
@@ -328,13 +327,15 @@ static void launch_march(hvr_ctx *c, const FrameConst &fc)
         }
 #undef HVR_CASE
     }
-    launch_march_t<TF_NONE, NM_ANY, 3, COUNT>(c, fc, ts, tn, ntx, nblocks, nullptr);
+    launch_march_t<TF_NONE, NM_ANY, 3, COUNT>(c, fc, ts, tn, ntx, nblocks, nullptr, out);
 }
 
 static int enqueue_frame(hvr_ctx *c, bool count, uchar4 *dst)
 {
     const FrameConst fc = frame_const(c);
     const int npix = c->W * c->H;
+    // events of this frame: the context's own, or the next slots of an
+    // open timing region; hvr_last_kernel_ms reads the frame's events
     hipEvent_t e0 = c->ev[0], e1 = c->ev[1], e2 = c->ev[2];
     if (c->tev && c->timing_n < c->timing_max) {
         const size_t k = 3 * (size_t)c->timing_n++;
@@ -342,22 +343,18 @@ static int enqueue_frame(hvr_ctx *c, bool count, uchar4 *dst)
         e1 = (*c->tev)[k + 1];
         e2 = (*c->tev)[k + 2];
     }
-    uchar4 *saved = c->d_out;
-    if (dst) c->d_out = dst;
     HIPCHK(hipEventRecord(e0, c->stream));
     hipLaunchKernelGGL(k_ray_setup, dim3((npix + 255) / 256), dim3(256), 0, c->stream, fc, c->d_rays);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(e1, c->stream));
-    if (count) launch_march<true>(c, fc);
-    else launch_march<false>(c, fc);
-    c->d_out = saved;
+    uchar4 *out = dst ? dst : c->d_out;
+    if (count) launch_march<true>(c, fc, out);
+    else launch_march<false>(c, fc, out);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(e2, c->stream));
-    if (e2 != c->ev[2]) {  // keep hvr_last_kernel_ms meaningful
-        HIPCHK(hipEventRecord(c->ev[0], c->stream));
-        HIPCHK(hipEventRecord(c->ev[1], c->stream));
-        HIPCHK(hipEventRecord(c->ev[2], c->stream));
-    }
+    c->last_ev[0] = e0;
+    c->last_ev[1] = e1;
+    c->last_ev[2] = e2;
     c->events_recorded = true;
     return HVR_OK;
 }
@@ -406,6 +403,23 @@ extern "C" {
 
 int hvr_abi_version(void) { return HVR_ABI_VERSION; }
 
+int hvr_debug_inject_fault(const char *function, int code, int after)
+{
+    std::lock_guard<std::mutex> lock(g_fault_mu);
+    if (!function) {
+        g_fault_armed.store(false);
+        g_fault_after = -1;
+        g_fault_fn.clear();
+        return HVR_OK;
+    }
+    if (code >= 0 || after < 0) return fail(HVR_EINVAL, "injected faults need a negative code and after >= 0");
+    g_fault_fn = function;
+    g_fault_code = code;
+    g_fault_after = after;
+    g_fault_armed.store(true);
+    return HVR_OK;
+}
+
 const char *hvr_last_error(void) { return g_err.c_str(); }
 
 int hvr_device_count(int *count)
@@ -427,6 +441,7 @@ int hvr_device_memory(int device, unsigned long long *free_bytes, unsigned long 
 
 int hvr_create(int device, int width, int height, hvr_ctx **out)
 {
+    HVR_FAULT_POINT("hvr_create");
     if (!out) return fail(HVR_EINVAL, "out is null");
     *out = nullptr;
     if (width <= 0 || height <= 0 || (long long)width * height > (1ll << 28))
@@ -443,6 +458,7 @@ int hvr_create(int device, int width, int height, hvr_ctx **out)
     c->force_global = gather && strcmp(gather, "global") == 0;
     const size_t npix = (size_t)width * height;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device);
     if (e == hipSuccess) e = hipMalloc(&c->d_out, npix * sizeof(uchar4));
     if (e == hipSuccess) e = hipMalloc(&c->d_rays, npix * sizeof(RayRec));
     if (e == hipSuccess) e = hipHostMalloc(&c->h_pinned, npix * 4, hipHostMallocDefault);
@@ -461,7 +477,36 @@ int hvr_create(int device, int width, int height, hvr_ctx **out)
         hvr_destroy(c);
         return fail_hip(e, "hvr_create");
     }
+    c->pix_cap = npix;
     *out = c;
+    return HVR_OK;
+}
+
+int hvr_resize(hvr_ctx *c, int width, int height)
+{
+    HVR_FAULT_POINT("hvr_resize");
+    if (!c) return fail(HVR_EINVAL, "ctx is null");
+    if (width <= 0 || height <= 0 || (long long)width * height > (1ll << 28))
+        return fail(HVR_EINVAL, "width and height must be positive (at most 2^28 pixels)");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const size_t npix = (size_t)width * height;
+    if (npix > c->pix_cap) {
+        (void)hipFree(c->d_out);
+        (void)hipFree(c->d_rays);
+        if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+        c->d_out = nullptr;
+        c->d_rays = nullptr;
+        c->h_pinned = nullptr;
+        c->pix_cap = 0;
+        HIPCHK(hipMalloc(&c->d_out, npix * sizeof(uchar4)));
+        HIPCHK(hipMalloc(&c->d_rays, npix * sizeof(RayRec)));
+        HIPCHK(hipHostMalloc(&c->h_pinned, npix * 4, hipHostMallocDefault));
+        HIPCHK(hipMemset(c->d_out, 0, npix * sizeof(uchar4)));
+        c->pix_cap = npix;
+    }
+    c->W = width;
+    c->H = height;
     return HVR_OK;
 }
 
@@ -551,6 +596,7 @@ static int texel_format(int sf, int nf)
 
 int hvr_volume_create(hvr_ctx *c, int s0, int s1, int s2, int sf, int nf, hvr_volume **out)
 {
+    HVR_FAULT_POINT("hvr_volume_create");
     if (!c || !out) return fail(HVR_EINVAL, "null argument");
     *out = nullptr;
     if (s0 <= 0 || s1 <= 0 || s2 <= 0) return fail(HVR_EINVAL, "volume dimensions must be positive");
@@ -690,6 +736,7 @@ static int volume_from_field(hvr_volume *v, const float *d_field, bool normals)
 
 int hvr_volume_upload_scalar_normals(hvr_volume *v, const float *data)
 {
+    HVR_FAULT_POINT("hvr_volume_upload_scalar_normals");
     if (!v || !data) return fail(HVR_EINVAL, "null argument");
     if (!fmt_has_scalar(v->fmt) || !fmt_has_normal(v->fmt))
         return fail(HVR_ESTATE, "the volume needs scalar and normal fields");
@@ -711,6 +758,7 @@ int hvr_volume_upload_scalar_normals(hvr_volume *v, const float *data)
 int hvr_volume_generate_blobs(hvr_volume *v, int n_blobs, const double *centers, const double *sigmas,
                               const double *amps, double noise, unsigned long long seed)
 {
+    HVR_FAULT_POINT("hvr_volume_generate_blobs");
     if (!v || (n_blobs > 0 && (!centers || !sigmas || !amps))) return fail(HVR_EINVAL, "null argument");
     if (n_blobs < 0 || n_blobs > kMaxBlobs) return fail(HVR_EINVAL, "0 to 32 blobs");
     if (!fmt_has_scalar(v->fmt)) return fail(HVR_ESTATE, "the volume has no scalar field");
@@ -751,6 +799,7 @@ int hvr_volume_generate_blobs(hvr_volume *v, int n_blobs, const double *centers,
 
 int hvr_volume_upload_scalar(hvr_volume *v, const float *data)
 {
+    HVR_FAULT_POINT("hvr_volume_upload_scalar");
     if (!v || !data) return fail(HVR_EINVAL, "null argument");
     if (!fmt_has_scalar(v->fmt)) return fail(HVR_ESTATE, "volume has no scalar field");
     int rc = upload_part(v, data, 1);
@@ -762,6 +811,7 @@ int hvr_volume_upload_scalar(hvr_volume *v, const float *data)
 
 int hvr_volume_upload_normals(hvr_volume *v, const float *data)
 {
+    HVR_FAULT_POINT("hvr_volume_upload_normals");
     if (!v || !data) return fail(HVR_EINVAL, "null argument");
     if (!fmt_has_normal(v->fmt)) return fail(HVR_ESTATE, "volume has no normal field");
     int rc = upload_part(v, data, 2);
@@ -794,27 +844,9 @@ int hvr_volume_destroy(hvr_volume *v)
     return HVR_OK;
 }
 
-// Prefix counts (n + 1 values) of LUT entries that are not exactly
-// transparent.  The march reads entry i as v[i] + a*(v[i+1] - v[i]) with a
-// in [0, 1), so entry i yields alpha 0 and a finite colour for every a (the
-// reach of a > 0 into entry i+1 is accounted for by the brick rule) only
-// when its alpha is 0 and the colours of it and of its successor, and the
-// successor's alpha, are bounded (|x| <= 1e38 also excludes NaN and
-// overflow of the difference: 0 * inf is NaN).
-static void lut_prefix_counts(const float *src, int cnt, int *nz)
-{
-    auto bounded = [](float x) { return fabsf(x) <= 1e38f; };
-    nz[0] = 0;
-    for (int i = 0; i < cnt; ++i) {
-        const float *e = src + 4 * i, *f = i + 1 < cnt ? e + 4 : e;
-        const bool clear = e[3] == 0.f && bounded(e[0]) && bounded(e[1]) && bounded(e[2]) && bounded(f[0]) &&
-                           bounded(f[1]) && bounded(f[2]) && bounded(f[3]);
-        nz[i + 1] = nz[i] + (clear ? 0 : 1);
-    }
-}
-
 int hvr_set_lut(hvr_ctx *c, const float *rgba, int n)
 {
+    HVR_FAULT_POINT("hvr_set_lut");
     if (!c) return fail(HVR_EINVAL, "ctx is null");
     if (n < 0 || (n > 0 && !rgba)) return fail(HVR_EINVAL, "bad LUT");
     HIPCHK(hipSetDevice(c->device));
@@ -857,6 +889,7 @@ static int check_params(const hvr_params *p)
 
 int hvr_set_params(hvr_ctx *c, const hvr_params *p)
 {
+    HVR_FAULT_POINT("hvr_set_params");
     if (!c || !p) return fail(HVR_EINVAL, "null argument");
     if (const int rc = check_params(p)) return rc;
     c->params = *p;
@@ -866,6 +899,7 @@ int hvr_set_params(hvr_ctx *c, const hvr_params *p)
 
 int hvr_bind(hvr_ctx *c, hvr_volume *s, int srole, hvr_volume *n, int nrole)
 {
+    HVR_FAULT_POINT("hvr_bind");
     if (!c) return fail(HVR_EINVAL, "ctx is null");
     if ((s && s->ctx != c) || (n && n->ctx != c)) return fail(HVR_EINVAL, "volume belongs to another context");
     if (srole != HVR_ROLE_SCALAR && srole != HVR_ROLE_NORMAL) return fail(HVR_EINVAL, "bad scalar role");
@@ -886,6 +920,7 @@ int hvr_bind(hvr_ctx *c, hvr_volume *s, int srole, hvr_volume *n, int nrole)
 
 int hvr_render_async(hvr_ctx *c)
 {
+    HVR_FAULT_POINT("hvr_render_async");
     if (!c) return fail(HVR_EINVAL, "ctx is null");
     HIPCHK(hipSetDevice(c->device));
     return enqueue_frame(c, false, nullptr);
@@ -893,6 +928,7 @@ int hvr_render_async(hvr_ctx *c)
 
 int hvr_render_async_to(hvr_ctx *c, void *dst)
 {
+    HVR_FAULT_POINT("hvr_render_async_to");
     if (!c) return fail(HVR_EINVAL, "ctx is null");
     HIPCHK(hipSetDevice(c->device));
     return enqueue_frame(c, false, (uchar4 *)dst);
@@ -978,6 +1014,7 @@ int hvr_timing_end(hvr_ctx *c, double *march_ms, double *setup_ms, int *frames)
 // costs more than the overlap saves there.
 int hvr_read(hvr_ctx *c, uint8_t *out)
 {
+    HVR_FAULT_POINT("hvr_read");
     if (!c || !out) return fail(HVR_EINVAL, "null argument");
     HIPCHK(hipSetDevice(c->device));
     const size_t bytes = (size_t)c->W * c->H * 4;
@@ -1036,10 +1073,17 @@ int hvr_framebuffer(hvr_ctx *c, void **ptr)
 
 int hvr_sample_counts(hvr_ctx *c, uint32_t *counts)
 {
+    HVR_FAULT_POINT("hvr_sample_counts");
     if (!c || !counts) return fail(HVR_EINVAL, "null argument");
     HIPCHK(hipSetDevice(c->device));
     const size_t npix = (size_t)c->W * c->H;
-    if (!c->d_counts) HIPCHK(hipMalloc(&c->d_counts, npix * sizeof(uint32_t)));
+    if (c->counts_cap < npix) {
+        (void)hipFree(c->d_counts);
+        c->d_counts = nullptr;
+        c->counts_cap = 0;
+        HIPCHK(hipMalloc(&c->d_counts, npix * sizeof(uint32_t)));
+        c->counts_cap = npix;
+    }
     int rc = enqueue_frame(c, true, nullptr);
     if (rc != HVR_OK) return rc;
     HIPCHK(hipMemcpyAsync(counts, c->d_counts, npix * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
@@ -1052,10 +1096,10 @@ int hvr_last_kernel_ms(hvr_ctx *c, float *march_ms, float *setup_ms)
     if (!c) return fail(HVR_EINVAL, "ctx is null");
     if (!c->events_recorded) return fail(HVR_ESTATE, "no frame rendered yet");
     HIPCHK(hipSetDevice(c->device));
-    HIPCHK(hipEventSynchronize(c->ev[2]));
+    HIPCHK(hipEventSynchronize(c->last_ev[2]));
     float a = 0.f, b = 0.f;
-    HIPCHK(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
-    HIPCHK(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+    HIPCHK(hipEventElapsedTime(&a, c->last_ev[0], c->last_ev[1]));
+    HIPCHK(hipEventElapsedTime(&b, c->last_ev[1], c->last_ev[2]));
     if (setup_ms) *setup_ms = a;
     if (march_ms) *march_ms = b;
     return HVR_OK;
@@ -1100,6 +1144,10 @@ struct hvr_batch {
     size_t out_pixels;
     int ncu;
 };
+
+// The serving layer's memory budget bounds a staged request by 512 bytes of
+// descriptor (pyvr_amd/hip_renderer/serve.py, batch_pool_bytes).
+static_assert(sizeof(ReqDesc) <= 512, "request descriptor above the serving budget's bound");
 
 typedef int (*BatchLaunchFn)(const char *blk, int cls, uint32_t *ctr, int lut_cap, int ncu, hipStream_t st);
 
@@ -1207,6 +1255,7 @@ extern "C" {
 
 int hvr_batch_create(hvr_ctx *c, hvr_batch **out)
 {
+    HVR_FAULT_POINT("hvr_batch_create");
     if (!c || !out) return fail(HVR_EINVAL, "null argument");
     *out = nullptr;
     HIPCHK(hipSetDevice(c->device));
@@ -1270,6 +1319,7 @@ int hvr_batch_set_tile_shape(hvr_batch *b, int shape)
 int hvr_batch_add(hvr_batch *b, hvr_volume *s, int srole, hvr_volume *n, int nrole, const float *lut_rgba,
                   int lut_n, const hvr_params *p, int width, int height, int *index)
 {
+    HVR_FAULT_POINT("hvr_batch_add");
     if (!b || !p) return fail(HVR_EINVAL, "null argument");
     if (!s || s->ctx != b->ctx || (n && n->ctx != b->ctx))
         return fail(HVR_EINVAL, "a batch request needs a scalar volume of the batch's context");
@@ -1303,6 +1353,7 @@ int hvr_batch_add(hvr_batch *b, hvr_volume *s, int srole, hvr_volume *n, int nro
 
 int hvr_batch_run_async(hvr_batch *b, int count)
 {
+    HVR_FAULT_POINT("hvr_batch_run_async");
     if (!b) return fail(HVR_EINVAL, "batch is null");
     hvr_ctx *c = b->ctx;
     HIPCHK(hipSetDevice(c->device));
@@ -1452,6 +1503,7 @@ int hvr_batch_run_async(hvr_batch *b, int count)
 
 int hvr_batch_wait(hvr_batch *b)
 {
+    HVR_FAULT_POINT("hvr_batch_wait");
     if (!b) return fail(HVR_EINVAL, "batch is null");
     HIPCHK(hipSetDevice(b->ctx->device));
     HIPCHK(hipStreamSynchronize(b->stream));
@@ -1460,6 +1512,7 @@ int hvr_batch_wait(hvr_batch *b)
 
 int hvr_batch_query(hvr_batch *b, int *done)
 {
+    HVR_FAULT_POINT("hvr_batch_query");
     if (!b || !done) return fail(HVR_EINVAL, "null argument");
     if (!b->ran) {
         *done = 1;
@@ -1505,6 +1558,74 @@ int hvr_batch_last_ms(hvr_batch *b, float *ms)
     HIPCHK(hipEventSynchronize(b->ev1));
     HIPCHK(hipEventElapsedTime(ms, b->ev0, b->ev1));
     return HVR_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Volume read-back
+
+extern "C" {
+
+int hvr_volume_download(hvr_volume *v, float *scalar, float *normals)
+{
+    if (!v || (!scalar && !normals)) return fail(HVR_EINVAL, "null argument");
+    if (scalar && (!fmt_has_scalar(v->fmt) || !v->scalar_loaded))
+        return fail(HVR_ESTATE, "the volume holds no uploaded scalar field");
+    if (normals && (!fmt_has_normal(v->fmt) || !v->normals_loaded))
+        return fail(HVR_ESTATE, "the volume holds no uploaded normal field");
+    hvr_ctx *c = v->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    const uint64_t plane = (uint64_t)v->s0 * v->s1;
+    // whole z-planes per chunk, up to 256 MiB of floats (scalar + normals)
+    const uint64_t per_voxel = (scalar ? 4 : 0) + (normals ? 12 : 0);
+    uint64_t zper = ((uint64_t)256 << 20) / (plane * per_voxel);
+    zper = std::max<uint64_t>(1, std::min<uint64_t>(zper, (uint64_t)v->s2));
+    float *d_s = nullptr, *d_n = nullptr;
+    int rc = HVR_OK;
+    if (scalar && hipMalloc(&d_s, zper * plane * 4) != hipSuccess) rc = fail(HVR_ENOMEM, "download staging");
+    if (rc == HVR_OK && normals && hipMalloc(&d_n, zper * plane * 12) != hipSuccess)
+        rc = fail(HVR_ENOMEM, "download staging");
+    for (uint64_t z0 = 0; rc == HVR_OK && z0 < (uint64_t)v->s2; z0 += zper) {
+        const uint64_t nz = std::min<uint64_t>(zper, (uint64_t)v->s2 - z0);
+        const uint64_t nvox = nz * plane;
+        const unsigned nblk = (unsigned)((nvox + 255) / 256);
+#define HVR_DL(F)                                                                                          \
+    case F:                                                                                                \
+        hipLaunchKernelGGL(k_download<F>, dim3(nblk), dim3(256), 0, c->stream, (const char *)v->d_texels,  \
+                           v->s0, v->s1, v->lb, v->nbx, v->nby, v->brick_bytes, (int)z0, nvox, d_s, d_n); \
+        break;
+        switch (v->fmt) {
+            HVR_DL(TF_S_F32)
+            HVR_DL(TF_S_F16)
+            HVR_DL(TF_S_U8)
+            HVR_DL(TF_N_F32)
+            HVR_DL(TF_N_SN8)
+            HVR_DL(TF_C_F32_F32)
+            HVR_DL(TF_C_F16_F32)
+            HVR_DL(TF_C_U8_F32)
+            HVR_DL(TF_C_F32_SN8)
+            HVR_DL(TF_C_F16_SN8)
+            HVR_DL(TF_C_U8_SN8)
+            HVR_DL(TF_C_F16_F16)
+            HVR_DL(TF_N_F16)
+        default: rc = fail(HVR_EINVAL, "bad texel format");
+        }
+#undef HVR_DL
+        if (rc != HVR_OK) break;
+        if (hipGetLastError() != hipSuccess) rc = fail(HVR_EDEVICE, "download kernel launch failed");
+        if (rc == HVR_OK && scalar &&
+            hipMemcpyAsync(scalar + z0 * plane, d_s, nvox * 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+            rc = fail(HVR_EDEVICE, "download copy failed");
+        if (rc == HVR_OK && normals &&
+            hipMemcpyAsync(normals + 3 * z0 * plane, d_n, nvox * 12, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+            rc = fail(HVR_EDEVICE, "download copy failed");
+        if (rc == HVR_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(HVR_EDEVICE, "download wait");
+    }
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(d_s);
+    (void)hipFree(d_n);
+    return rc;
 }
 
 }  // extern "C"

@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "hvr_rules.h"
+
 namespace hvr {
 
 // ---------------------------------------------------------------------------
@@ -94,7 +96,6 @@ struct TexDesc {
     int role;              // field a sampler reads: 0 scalar, 1 normal
 };
 
-__host__ __device__ constexpr uint32_t bricks_along(int s, int lb) { return ((uint32_t)s >> lb) + 1u; }
 
 // Loads and stores through pointers the compiler cannot place (fetched from
 // descriptors in memory): they address device global memory, so issue
@@ -990,14 +991,7 @@ k_brick_range(const char *__restrict__ texels, uint32_t brick_bytes, uint32_t nb
 // transparent bricks (one atomic per wave).
 __device__ __forceinline__ bool brick_clear(float2 r, const int *__restrict__ nz, int n)
 {
-    const float fn = (float)n, fn1 = (float)(n - 1);
-    // an interpolated density is off by a few ulps of the largest tap
-    const float m = fmaxf(fabsf(r.x), fabsf(r.y)) * fn * 1e-5f + 1e-3f;
-    const float ulo = r.x * fn - 0.5f - m;
-    const float uhi = r.y * fn - 0.5f + m;
-    const int i0 = (int)floorf(fminf(fmaxf(ulo, 0.f), fn1));
-    const int i1 = (int)ceilf(fminf(fmaxf(uhi, 0.f), fn1));
-    return nz[i1 + 1] - nz[i0] == 0;
+    return brick_clear_range(r.x, r.y, nz, n);
 }
 
 __global__ void __launch_bounds__(256)
@@ -1030,34 +1024,6 @@ __device__ __forceinline__ uint32_t enc_u8(float v)
 {
     v = fminf(fmaxf(v, 0.f), 1.f);
     return (uint32_t)rintf(v * 255.f);
-}
-
-// Slots along one axis that hold texel coordinate c of a texture of size s
-// stored in nb bricks: (brick, local) pairs; returns how many (1..4).  A
-// position whose own brick would be brick nb (X = s+1 with s+1 a multiple
-// of B) is only ever read through the apron of brick nb-1.
-__device__ __forceinline__ int axis_slots(int c, int s, int lb, uint32_t nb, uint32_t br[4], uint32_t lo[4])
-{
-    const uint32_t B = 1u << lb, M = B - 1u;
-    int n = 0;
-    uint32_t X[3];
-    int nx = 0;
-    X[nx++] = (uint32_t)c + 1u;
-    if (c == 0) X[nx++] = 0u;
-    if (c == s - 1) X[nx++] = (uint32_t)s + 1u;
-    for (int k = 0; k < nx; ++k) {
-        if ((X[k] >> lb) < nb) {
-            br[n] = X[k] >> lb;
-            lo[n] = X[k] & M;
-            ++n;
-        }
-        if ((X[k] & M) == 0u && X[k] > 0u) {  // apron copy in the previous brick
-            br[n] = (X[k] >> lb) - 1u;
-            lo[n] = B;
-            ++n;
-        }
-    }
-    return n;
 }
 
 template <int F>
@@ -1375,6 +1341,34 @@ __global__ void __launch_bounds__(256) k_batch_flags(const char *__restrict__ bl
 __global__ void __launch_bounds__(64) k_fill_u32(uint32_t *__restrict__ p, int n, uint32_t v)
 {
     for (int i = threadIdx.x; i < n; i += 64) p[i] = v;
+}
+
+// Decoded texels of a volume in numpy C order (texel (x, y, z) at flat
+// x + s0*(y + s1*z)), z-planes z0 .. z0 + nz - 1: the scalar as the march
+// reads it (UNORM8 times 1/255) and the normal as stored (SNORM8 as its
+// integer, FP16 as float).  Either output may be null.
+template <int F>
+__global__ void __launch_bounds__(256)
+k_download(const char *__restrict__ texels, int s0, int s1, int lb, uint32_t nbx, uint32_t nby, uint32_t brick_bytes,
+           int z0, uint64_t nvox, float *__restrict__ scalar, float *__restrict__ normal)
+{
+    const uint64_t l = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (l >= nvox) return;
+    const uint64_t plane = (uint64_t)s0 * (uint64_t)s1;
+    const uint32_t z = (uint32_t)(l / plane) + (uint32_t)z0;
+    const uint64_t rem = l % plane;
+    const uint32_t y = (uint32_t)(rem / (uint64_t)s0), x = (uint32_t)(rem % (uint64_t)s0);
+    // the texel's own slot in the extended (apron) coordinates X = x + 1
+    const uint32_t X = x + 1u, Y = y + 1u, Z = z + 1u, M = (1u << lb) - 1u, E = (1u << lb) + 1u;
+    const uint64_t brick = ((uint64_t)(Z >> lb) * nby + (Y >> lb)) * nbx + (X >> lb);
+    const uint32_t local = ((Z & M) * E + (Y & M)) * E + (X & M);
+    const float4 t = fetch4<F>(texels + brick * brick_bytes + (uint64_t)local * (uint32_t)texel_bytes(F));
+    if (scalar) scalar[l] = fmt_u8_scalar(F) ? t.w * (1.0f / 255.0f) : t.w;
+    if (normal) {
+        normal[3 * l] = t.x;
+        normal[3 * l + 1] = t.y;
+        normal[3 * l + 2] = t.z;
+    }
 }
 
 }  // namespace hvr

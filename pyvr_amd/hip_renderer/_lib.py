@@ -41,7 +41,8 @@ EXPORTS = (
     "hvr_skip_info", "hvr_batch_create", "hvr_batch_destroy", "hvr_batch_clear", "hvr_batch_set_tile_shape",
     "hvr_batch_add", "hvr_batch_run_async", "hvr_batch_wait", "hvr_batch_run", "hvr_batch_output",
     "hvr_batch_counts", "hvr_batch_last_ms", "hvr_volume_upload_scalar_normals", "hvr_volume_generate_blobs",
-    "hvr_batch_query", "hvr_device_memory", "hvr_read_pinned",
+    "hvr_batch_query", "hvr_device_memory", "hvr_read_pinned", "hvr_volume_download",
+    "hvr_resize", "hvr_debug_inject_fault",
 )
 
 # batch ray-tile shapes (hvr_batch_set_tile_shape)
@@ -85,10 +86,13 @@ def _declare(L):
         "hvr_device_count": ([ctypes.POINTER(i)], i),
         "hvr_create": ([i, i, i, ctypes.POINTER(vp)], i),
         "hvr_destroy": ([vp], i),
+        "hvr_resize": ([vp, i, i], i),
+        "hvr_debug_inject_fault": ([ctypes.c_char_p, i, i], i),
         "hvr_volume_create": ([vp, i, i, i, i, i, ctypes.POINTER(vp)], i),
         "hvr_volume_upload_scalar": ([vp, fp], i),
         "hvr_volume_upload_normals": ([vp, fp], i),
         "hvr_volume_bytes": ([vp, ctypes.POINTER(ctypes.c_uint64)], i),
+        "hvr_volume_download": ([vp, fp, fp], i),
         "hvr_volume_destroy": ([vp], i),
         "hvr_set_lut": ([vp, fp, i], i),
         "hvr_set_params": ([vp, ctypes.POINTER(HvrParams)], i),
@@ -168,3 +172,12 @@ def check(rc: int, what: str = "") -> None:
     if rc != HVR_OK:
         msg = lib().hvr_last_error().decode(errors="replace")
         raise RuntimeError(f"{what} failed ({rc}): {msg}" if what else f"hipvr error {rc}: {msg}")
+
+
+def inject_fault(function, code: int = -2, after: int = 0) -> None:
+    """Make the library entry point ``function`` fail with ``code`` on its
+    ``after``-th call from now, once (``hvr_debug_inject_fault``); ``None``
+    disarms.  A testing aid: the failure surfaces as the RuntimeError that
+    ``check`` raises, with the message "injected fault in <function>"."""
+    fn = None if function is None else function.encode()
+    check(lib().hvr_debug_inject_fault(fn, int(code), int(after)), "hvr_debug_inject_fault")

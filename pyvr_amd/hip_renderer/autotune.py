@@ -23,9 +23,15 @@ BRICK_LOG2 = (3, 4)
 def signature(volumes: Sequence, requests: Sequence, scalar_format: str, normal_format: str) -> str:
     """Workload key: formats, the volumes' shapes (with/without normals) and
     the requests' (volume, width, height) multiset."""
-    vols = [f"{'x'.join(map(str, v.data.shape))}{'n' if v.has_normals else ''}" for v in volumes]
+    vols = [f"{'x'.join(map(str, _shape(v)))}{'n' if v.has_normals else ''}" for v in volumes]
     reqs = sorted(f"{q.volume}:{q.width}x{q.height}" for q in requests)
     return f"{scalar_format}+{normal_format}|{','.join(vols)}|{len(requests)}|{';'.join(reqs)}"
+
+
+def _shape(volume):
+    """Shape of a ``Volume`` (its data) or of a device-generated volume."""
+    data = getattr(volume, "data", None)
+    return tuple(data.shape) if data is not None else tuple(volume.shape)
 
 
 def _sample(requests: Sequence, max_requests: int) -> List:

@@ -25,7 +25,7 @@ from ..camera import Camera
 from ..config import REFERENCE_TERMINATION_ALPHA, RenderConfig
 from ..lighting import Light
 from ..volume import Volume
-from . import _lib
+from . import _lib, trace
 from ._lib import HvrParams, check
 from .manager import _FORMAT_ALIASES, _DeviceVolume
 
@@ -302,7 +302,8 @@ class BatchRenderer:
 
     def _launch(self, k: int, count: bool):
         s = self._slots[k]
-        check(self.L.hvr_batch_run_async(s.handle, int(count)), "hvr_batch_run_async")
+        with trace.span("hvr.batch.launch"):
+            check(self.L.hvr_batch_run_async(s.handle, int(count)), "hvr_batch_run_async")
         s.in_flight = True
         self._last = k
 
@@ -312,7 +313,8 @@ class BatchRenderer:
         if k is None:
             return
         s = self._slots[k]
-        check(self.L.hvr_batch_wait(s.handle), "hvr_batch_wait")
+        with trace.span("hvr.batch.wait"):
+            check(self.L.hvr_batch_wait(s.handle), "hvr_batch_wait")
         s.in_flight = False
 
     def done(self, slot: int) -> bool:
@@ -369,7 +371,8 @@ class BatchRenderer:
             self.wait(k)
         ptr = ctypes.POINTER(ctypes.c_uint8)()
         check(self.L.hvr_batch_output(s.handle, index, ctypes.byref(ptr)), "hvr_batch_output")
-        return ctypes.string_at(ptr, w * h * 4)
+        with trace.span("hvr.batch.readback"):
+            return ctypes.string_at(ptr, w * h * 4)
 
     def results(self, slot: Optional[int] = None) -> List[bytes]:
         k, s = self._slot(slot)

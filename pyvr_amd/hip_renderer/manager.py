@@ -137,6 +137,18 @@ class _DeviceVolume:
         check(self.L.hvr_volume_bytes(self.handle, ctypes.byref(v)), "hvr_volume_bytes")
         return int(v.value)
 
+    def download(self, scalar: bool = True, normals: bool = False):
+        """The stored fields as float32 numpy arrays of the volume's shape
+        (``hvr_volume_download``): the scalar as the march reads it, the
+        normals as stored (SNORM8 as integers).  Returns (scalar, normals),
+        None for a field not asked for."""
+        fp = ctypes.POINTER(ctypes.c_float)
+        s = np.empty(self.shape, np.float32) if scalar else None
+        n = np.empty(self.shape + (3,), np.float32) if normals else None
+        check(self.L.hvr_volume_download(self.handle, s.ctypes.data_as(fp) if s is not None else None,
+                                         n.ctypes.data_as(fp) if n is not None else None), "hvr_volume_download")
+        return s, n
+
     def destroy(self):
         if self.handle:
             self.L.hvr_volume_destroy(self.handle)

@@ -21,6 +21,7 @@ from typing import Optional
 
 import numpy as np
 
+from . import trace
 from ..camera import Camera
 from ..config import REFERENCE_TERMINATION_ALPHA, RenderConfig
 from ..lighting import Light
@@ -131,10 +132,11 @@ class HipVolumeRenderer:
 
     def render(self) -> bytes:
         """Render one frame: width*height*4 bytes, RGBA8, bottom row first."""
-        self.gl_manager.clear_framebuffer(0.0, 0.0, 0.0, 0.0)
-        self.gl_manager.setup_blending()
-        self.gl_manager.render_quad()
-        return self.gl_manager.read_pixels()
+        with trace.span("hvr.render"):
+            self.gl_manager.clear_framebuffer(0.0, 0.0, 0.0, 0.0)
+            self.gl_manager.setup_blending()
+            self.gl_manager.render_quad()
+            return self.gl_manager.read_pixels()
 
     def render_to_pil(self, data=None):
         """Render (or take ``data``) and return a top-row-first PIL image."""

@@ -14,7 +14,12 @@ turns a stream of requests into batches for ``BatchRenderer``:
 * a volume is uploaded (or generated) on first use and stays resident;
   when the budget would be exceeded, resident volumes that no queued-for-
   launch or in-flight batch uses are evicted, least recently used first;
-  the budget also reserves the batches' output pools;
+  the budget also reserves what the batches in flight hold (output
+  pools, per-request brick flags for the largest registered volume,
+  staging), as those pools grow;
+* with ``max_batch_samples`` a batch also stops at that many estimated
+  samples (``estimated_samples``), so that a burst of large requests is
+  split into batches of bounded duration;
 * a request's latency runs from its arrival to its pixels being in host
   memory; ``poll()`` returns the requests that completed, with ``rgba``
   filled (the same bytes ``VolumeRenderer.render()`` gives for it) unless
@@ -23,17 +28,26 @@ turns a stream of requests into batches for ``BatchRenderer``:
 
 import time
 from collections import OrderedDict, deque
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 from typing import Deque, Dict, Hashable, List, Optional
 
 import numpy as np
 
-from . import _lib
+from . import _lib, trace
 from .batch import BatchRenderer, DeviceBlobVolume
 
 _TEXEL_BYTES = {("f32", "none"): 4, ("f16", "none"): 2, ("u8", "none"): 1,
                 ("f32", "f32x3"): 16, ("f16", "f32x3"): 16, ("u8", "f32x3"): 16,
                 ("f32", "sn8x3"): 8, ("f16", "sn8x3"): 8, ("u8", "sn8x3"): 4, ("f16", "f16x3"): 8}
+
+
+def volume_bricks(shape, brick_log2: int = 3) -> int:
+    """Bricks of a volume: per axis (s >> brick_log2) + 1 (the extended
+    texture holds s + 2 texels, the edge copies included)."""
+    nb = 1
+    for s in shape:
+        nb *= (int(s) >> brick_log2) + 1
+    return nb
 
 
 def volume_device_bytes(shape, scalar_format: str, normal_format: str, has_normals: bool,
@@ -43,10 +57,69 @@ def volume_device_bytes(shape, scalar_format: str, normal_format: str, has_norma
     nf = normal_format if has_normals else "none"
     t = _TEXEL_BYTES[scalar_format, nf]
     e = (1 << brick_log2) + 1
-    nb = 1
-    for s in shape:
-        nb *= (int(s) >> brick_log2) + 1
+    nb = volume_bricks(shape, brick_log2)
     return nb * e ** 3 * t + nb * 8
+
+
+# Upper bounds of what one batch of hvr_batch_run_async stages per request:
+# descriptor (below 512 bytes), LUT of at most 2048 RGBA float entries and
+# its prefix counts, each block aligned to 256 bytes.
+_DESC_BYTES = 512
+_MAX_LUT = 2048
+
+
+def _pool(need: int) -> int:
+    """Bytes a batch pool holds for ``need``: pools start at 4 KiB and
+    double until they fit (never shrink)."""
+    n = 4096
+    while n < need:
+        n *= 2
+    return n
+
+
+def batch_pool_bytes(max_requests: int, max_pixels: int, max_bricks: int) -> int:
+    """Device bytes one batch slot holds at most for batches of up to
+    ``max_requests`` requests and ``max_pixels`` pixels over volumes of up
+    to ``max_bricks`` bricks: the RGBA8 output pool (each request padded to
+    64 pixels), the transparent-brick flags (one byte per brick per request,
+    padded to 256), the staged block (header, descriptors, LUTs, prefix
+    counts) and the tile counters; each pool rounded up as it grows."""
+    out = 4 * (max_pixels + 64 * max_requests)
+    flags = max_requests * (-(-max_bricks // 256) * 256)
+    stage = 256 + max_requests * (_DESC_BYTES + 16 * _MAX_LUT + 4 * (_MAX_LUT + 1) + 3 * 256)
+    return _pool(out) + _pool(flags) + _pool(stage) + 256
+
+
+def estimated_samples(width: int, height: int, camera, config, bounds=None) -> float:
+    """A cheap estimate of a request's reference samples, to bound a batch's
+    work: pixels whose rays can hit the box (the bounding sphere's image,
+    from the camera distance and the field of view) times the samples of a
+    mean chord (4 V / S of the box) at the step size, capped by max_steps.
+    Termination and empty space are ignored: it orders and bounds requests,
+    it is not a count."""
+    import math
+
+    lo = np.asarray(bounds[0] if bounds is not None else (-1.0, -1.0, -1.0), np.float64)
+    hi = np.asarray(bounds[1] if bounds is not None else (1.0, 1.0, 1.0), np.float64)
+    ext = np.maximum(hi - lo, 1e-12)
+    centre = 0.5 * (lo + hi)
+    radius = 0.5 * float(np.linalg.norm(ext))
+    try:
+        pos = np.asarray(camera.get_camera_vectors()[0], np.float64)
+        fov = float(getattr(camera, "fov", math.pi / 4))
+    except AttributeError:
+        pos, fov = centre + 3.0 * radius, math.pi / 4
+    d = float(np.linalg.norm(pos - centre))
+    if d <= radius:
+        frac = 1.0
+    else:
+        r_ndc = math.tan(math.asin(radius / d)) / math.tan(fov / 2)     # disc radius, NDC units of y
+        aspect = width / height
+        frac = min(1.0, math.pi * r_ndc * r_ndc / (4.0 * aspect))
+    chord = 4.0 * float(np.prod(ext)) / (2.0 * (ext[0] * ext[1] + ext[1] * ext[2] + ext[0] * ext[2]))
+    step = float(getattr(config, "step_size", 0.01)) if config is not None else 0.01
+    steps = float(getattr(config, "max_steps", 500)) if config is not None else 500.0
+    return width * height * frac * min(steps, chord / step)
 
 
 @dataclass
@@ -66,6 +139,8 @@ class ServeRequest:
     done: Optional[float] = None
     rgba: Optional[np.ndarray] = None  # width*height*4 uint8, bottom row first
     tag: object = None
+    estimate: Optional[float] = None  # estimated samples (set by the server when it bounds batches by work)
+    cancelled: bool = False           # set by RenderServer.cancel
 
     @property
     def latency(self) -> Optional[float]:
@@ -103,7 +178,7 @@ class RenderServer:
     def __init__(self, device: int = 0, scalar_format: str = "f16", normal_format: str = "sn8x3",
                  brick_log2: int = 3, tile_shape: str = "8x8", memory_budget: Optional[int] = None,
                  max_batch_requests: int = 256, max_batch_pixels: int = 1 << 25, pipeline: int = 3,
-                 keep_results: bool = True, backend=None):
+                 keep_results: bool = True, backend=None, max_batch_samples: Optional[float] = None):
         self.scalar_format, self.normal_format, self.brick_log2 = scalar_format, normal_format, brick_log2
         self.backend = backend if backend is not None else BatchRenderer(
             device=device, scalar_format=scalar_format, normal_format=normal_format, brick_log2=brick_log2,
@@ -114,17 +189,22 @@ class RenderServer:
         self.keep_results = keep_results
         self.max_batch_requests = max_batch_requests
         self.max_batch_pixels = max_batch_pixels
-        # output pools of the batches in flight: device RGBA8 per pixel
-        self.reserved = pipeline * max_batch_pixels * 4
+        self.max_batch_samples = max_batch_samples
+        self.pipeline = pipeline
+        # the pools of the batches in flight (outputs, brick flags, staging),
+        # for the largest registered volume's brick count
+        self.max_bricks = 0
+        self.reserved = self._reserve(0)
         self.memory_budget = int(memory_budget)
         if self.memory_budget <= self.reserved:
-            raise ValueError("memory budget does not cover the batches' output pools")
+            raise ValueError("memory budget does not cover the batches' pools")
         self._registry: Dict[Hashable, tuple] = {}
         self._resident: "OrderedDict[Hashable, _Resident]" = OrderedDict()
         self._queue: Deque[ServeRequest] = deque()
         self._flight: Dict[int, List[ServeRequest]] = {}
         self.batches = 0
         self.evictions = 0
+        self.cancelled = 0
 
     # -- volumes -------------------------------------------------------------
     def register(self, key: Hashable, volume, device_normals: bool = False):
@@ -134,9 +214,15 @@ class RenderServer:
         has_n = volume.has_normals or (device_normals and not isinstance(volume, DeviceBlobVolume))
         shape = volume.shape if isinstance(volume, DeviceBlobVolume) else volume.data.shape
         nbytes = volume_device_bytes(shape, self.scalar_format, self.normal_format, has_n, self.brick_log2)
-        if nbytes > self.memory_budget - self.reserved:
+        reserved = self._reserve(max(self.max_bricks, volume_bricks(shape, self.brick_log2)))
+        if nbytes > self.memory_budget - reserved:
             raise ValueError(f"volume {key!r} needs {nbytes} bytes, more than the budget allows")
+        self.max_bricks = max(self.max_bricks, volume_bricks(shape, self.brick_log2))
+        self.reserved = reserved
         self._registry[key] = (volume, device_normals, nbytes)
+
+    def _reserve(self, max_bricks: int) -> int:
+        return self.pipeline * batch_pool_bytes(self.max_batch_requests, self.max_batch_pixels, max_bricks)
 
     def preload(self, keys=None):
         """Load registered volumes now (as many as the budget holds)."""
@@ -185,6 +271,28 @@ class RenderServer:
     def pending(self) -> int:
         return len(self._queue) + sum(len(v) for v in self._flight.values())
 
+    def cancel(self, req: ServeRequest) -> bool:
+        """Withdraw a submitted request.  A queued request leaves the queue
+        and is never launched; a request already in a batch on the GPU runs
+        to the end of that batch (a launch is not interrupted), but its
+        pixels are discarded when the batch is collected.  Cancelled
+        requests are not returned by ``poll``.  Returns False when the
+        request had already completed (or was never submitted)."""
+        if req.cancelled:
+            return True
+        for i, r in enumerate(self._queue):
+            if r is req:
+                del self._queue[i]
+                req.cancelled = True
+                self.cancelled += 1
+                return True
+        for reqs in self._flight.values():
+            if any(r is req for r in reqs):
+                req.cancelled = True
+                self.cancelled += 1
+                return True
+        return False
+
     def poll(self) -> List[ServeRequest]:
         """Collect finished batches, then launch queued requests into free
         slots; never waits for the GPU.  Returns the completed requests."""
@@ -205,22 +313,45 @@ class RenderServer:
         return out
 
     def _collect(self, k) -> List[ServeRequest]:
+        with trace.span("hvr.serve.collect"):
+            return self._collect_slot(k)
+
+    def _collect_slot(self, k) -> List[ServeRequest]:
         now = time.perf_counter()
         reqs = self._flight.pop(k)
         for i, r in enumerate(reqs):
+            self._resident[r.volume].pins -= 1
+            if r.cancelled:
+                continue
             r.done = now
             if self.keep_results:
                 r.rgba = np.array(self.backend.view(i, k))
-            self._resident[r.volume].pins -= 1
-        return reqs
+        return [r for r in reqs if not r.cancelled]
+
+    def _estimate(self, r: ServeRequest) -> float:
+        if r.estimate is None:
+            vol = self._registry[r.volume][0]
+            lo, hi = getattr(vol, "min_bounds", None), getattr(vol, "max_bounds", None)
+            r.estimate = estimated_samples(r.width, r.height, r.camera, r.config,
+                                           None if lo is None or hi is None else (lo, hi))
+        return r.estimate
 
     def _launch(self) -> bool:
-        chosen, pixels, waiting = [], 0, deque()
+        with trace.span("hvr.serve.admit"):
+            return self._admit_and_launch()
+
+    def _admit_and_launch(self) -> bool:
+        chosen, pixels, work, waiting = [], 0, 0.0, deque()
         while self._queue and len(chosen) < self.max_batch_requests:
             r = self._queue[0]
             px = r.width * r.height
             if chosen and pixels + px > self.max_batch_pixels:
                 break
+            if self.max_batch_samples is not None:
+                est = self._estimate(r)
+                if chosen and work + est > self.max_batch_samples:
+                    break
+                work += est
             self._queue.popleft()
             # pin before loading others, so a request's volume is not evicted
             # by the next request of the same batch

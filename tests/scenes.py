@@ -175,6 +175,26 @@ class Request:
         return rgba
 
 
+def request_uniforms(q: Request, volume) -> dict:
+    """The oracle's uniform block for request ``q`` on ``volume`` (a
+    ``Volume`` or a device volume with bounds): what a fresh renderer of the
+    request's size sets after set_camera, set_light and
+    set_opacity_threshold."""
+    pos, _ = q.camera.get_camera_vectors()
+    return dict(
+        width=q.width, height=q.height,
+        view=q.camera.get_view_matrix(),
+        proj=q.camera.get_projection_matrix(q.width / q.height),
+        cam_pos=np.asarray(pos, dtype=np.float32),
+        bmin=np.asarray(volume.min_bounds, np.float32), bmax=np.asarray(volume.max_bounds, np.float32),
+        step=q.config.step_size, ref_step=q.config.reference_step_size, max_steps=q.config.max_steps,
+        ambient=q.light.ambient_intensity, diffuse=q.light.diffuse_intensity,
+        light_pos=np.asarray(q.light.position, np.float32),
+        light_target=np.asarray(q.light.target, np.float32),
+        et_threshold=q.tau,
+    )
+
+
 def c3_volumes(scale: int = 1, seed: int = 3, with_normals: bool = True, shapes=None):
     """The batch's volumes: blob fields (8 blobs, seed 10*seed + k) over
     [-1, 1]^3 with central-difference normals; sizes divided by ``scale``."""

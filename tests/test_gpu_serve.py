@@ -15,12 +15,14 @@ pytestmark = pytest.mark.gpu
 
 
 def test_served_results_equal_single_renders(hip):
-    from pyvr_amd.hip_renderer.serve import RenderServer, ServeRequest, volume_device_bytes
+    from pyvr_amd.hip_renderer.serve import (RenderServer, ServeRequest, batch_pool_bytes, volume_bricks,
+                                             volume_device_bytes)
 
     vols = scenes.c3_volumes(scale=4)
     sizes = sorted(volume_device_bytes(v.data.shape, "f16", "sn8x3", True) for v in vols)
     pipeline, max_px = 2, 1 << 16
-    budget = pipeline * max_px * 4 + sizes[-1] + sizes[-2] + 1      # two largest volumes at most
+    pools = pipeline * batch_pool_bytes(5, max_px, max(volume_bricks(v.data.shape) for v in vols))
+    budget = pools + sizes[-1] + sizes[-2] + 1      # two largest volumes at most
     srv = RenderServer(scalar_format="f16", normal_format="sn8x3", memory_budget=budget,
                        max_batch_requests=5, max_batch_pixels=max_px, pipeline=pipeline)
     try:

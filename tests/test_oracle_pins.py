@@ -117,6 +117,17 @@ def test_single_entry_lut():
         assert np.allclose(oracle.lut_sample(Q, rho), Q[0], atol=1e-7)
 
 
+def test_lut_nan_and_infinite_density():
+    """u = rho*N - 1/2 clamped with IEEE maxNum/minNum (the kernels' fmaxf /
+    fminf): a NaN density reads entry 0 exactly, +inf the last entry, -inf
+    the first."""
+    n = 5
+    Q = np.arange(4 * n, dtype=np.float32).reshape(n, 4) + 1
+    assert np.array_equal(oracle.lut_sample(Q, float("nan")), Q[0])
+    assert np.array_equal(oracle.lut_sample(Q, float("inf")), Q[-1])
+    assert np.array_equal(oracle.lut_sample(Q, float("-inf")), Q[0])
+
+
 # ---------------------------------------------------------------------------
 # blend / quantise
 def _round_half_up(fr: Fraction) -> int:

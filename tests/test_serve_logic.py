@@ -9,7 +9,8 @@ from types import SimpleNamespace
 import numpy as np
 import pytest
 
-from pyvr_amd.hip_renderer.serve import RenderServer, ServeRequest, volume_device_bytes
+from pyvr_amd.hip_renderer.serve import (RenderServer, ServeRequest, batch_pool_bytes, volume_bricks,
+                                         volume_device_bytes)
 
 
 class FakeBackend:
@@ -79,8 +80,9 @@ def test_footprint_rule():
 
 def make(budget_volumes, pipeline=2, **kw):
     be = FakeBackend(pipeline)
-    reserve = pipeline * kw.get("max_batch_pixels", 1 << 12) * 4
     kw.setdefault("max_batch_pixels", 1 << 12)
+    kw.setdefault("max_batch_requests", 256)
+    reserve = pipeline * batch_pool_bytes(kw["max_batch_requests"], kw["max_batch_pixels"], volume_bricks((16,) * 3))
     srv = RenderServer(backend=be, memory_budget=reserve + budget_volumes * B16 + 1, pipeline=pipeline, **kw)
     for k in "ABC":
         srv.register(k, vol(k, 16))
@@ -175,7 +177,7 @@ class LateBackend(FakeBackend):
 @pytest.mark.parametrize("after", [0, 1, 2, 3])
 def test_no_request_lost_when_a_batch_finishes_mid_poll(after):
     be = LateBackend(2, after)
-    srv = RenderServer(backend=be, memory_budget=2 * (1 << 12) * 4 + 3 * B16 + 1, pipeline=2,
+    srv = RenderServer(backend=be, memory_budget=2 * batch_pool_bytes(2, 1 << 12, 27) + 3 * B16 + 1, pipeline=2,
                        max_batch_requests=2, max_batch_pixels=1 << 12)
     for k in "ABC":
         srv.register(k, vol(k, 16))
@@ -220,3 +222,102 @@ def test_camera_path_requests_are_served_in_order():
     assert [r.tag[1] for r in done] == list(range(7))
     moves = CameraController().animate_to(Camera.isometric_view(), n_frames=5)
     assert all(r.arrival is None for r in path_requests("B", moves, None, None))
+
+
+def test_pool_reservation_covers_what_a_batch_allocates():
+    """The budget's reservation per slot bounds what hvr_batch_run_async
+    grows its pools to (hipvr.hip: output pool of 64-pixel-padded images,
+    256-padded brick flags per request, staged block; 4 KiB start,
+    doubling), for random batches within the limits."""
+    rng = np.random.default_rng(5)
+
+    def grow(cap, need):
+        if need <= cap:
+            return cap
+        n = cap or 4096
+        while n < need:
+            n *= 2
+        return n
+
+    for _ in range(200):
+        max_req, max_px, max_b = int(rng.integers(1, 40)), int(rng.integers(1, 1 << 22)), int(rng.integers(1, 1 << 21))
+        out = flags = stage = 0
+        for _ in range(5):                      # successive batches on one slot
+            n = int(rng.integers(1, max_req + 1))
+            px = rng.integers(1, max(2, max_px // n), n)
+            bricks = rng.integers(1, max_b + 1, n)
+            luts = rng.integers(1, 2049, n)
+            out = grow(out, 4 * int(sum(-(-int(p) // 64) * 64 for p in px)))
+            flags = grow(flags, int(sum(-(-int(b) // 256) * 256 for b in bricks)))
+            desc_end = -(-(256 + n * 512) // 256) * 256
+            lut_end = -(-(desc_end + int(sum(16 * luts))) // 256) * 256
+            stage = grow(stage, -(-(lut_end + int(sum(4 * (luts + 1)))) // 256) * 256)
+        assert out + flags + stage + 256 <= batch_pool_bytes(max_req, max_px, max_b)
+
+
+def test_registering_a_larger_volume_raises_the_reservation():
+    srv = RenderServer(backend=FakeBackend(2), memory_budget=1 << 30, pipeline=2, max_batch_pixels=1 << 12)
+    srv.register("A", vol("A", 16))
+    before = srv.reserved
+    assert before == 2 * batch_pool_bytes(256, 1 << 12, volume_bricks((16,) * 3))
+    srv.register("D", vol("D", 64))
+    assert srv.reserved > before
+    assert srv.reserved == 2 * batch_pool_bytes(256, 1 << 12, volume_bricks((64,) * 3))
+    # a volume whose flags would push the reservation past the budget is refused
+    tight = RenderServer(backend=FakeBackend(2), memory_budget=before + 2 * B16, pipeline=2,
+                         max_batch_pixels=1 << 12)
+    tight.register("A", vol("A", 16))
+    with pytest.raises(ValueError):
+        tight.register("D", vol("D", 64))
+    assert tight.reserved == before
+
+
+def test_batches_bounded_by_estimated_samples():
+    """With max_batch_samples a burst of requests is split so that each
+    batch's estimated work stays within the bound (a single request larger
+    than the bound still launches alone)."""
+    from pyvr_amd.camera import Camera
+    from pyvr_amd.config import RenderConfig
+    from pyvr_amd.hip_renderer.serve import estimated_samples
+
+    cam = Camera.from_spherical(target=np.zeros(3, np.float32), azimuth=0.3, elevation=0.2, roll=0.0, distance=3.0)
+    cfg = RenderConfig.balanced()
+    one = estimated_samples(64, 64, cam, cfg)
+    assert 0 < one <= 64 * 64 * cfg.max_steps
+    # a larger image and a finer step cost more; a farther camera less
+    assert estimated_samples(128, 128, cam, cfg) == pytest.approx(4 * one)
+    assert estimated_samples(64, 64, cam, RenderConfig.high_quality()) > one
+    far = Camera.from_spherical(target=np.zeros(3, np.float32), azimuth=0.3, elevation=0.2, roll=0.0, distance=12.0)
+    assert estimated_samples(64, 64, far, cfg) < one
+    srv, be = make(3, max_batch_requests=16, max_batch_pixels=1 << 20, max_batch_samples=2.5 * one)
+    for k in "ABCABC":
+        r = req(k, 64, 64)
+        r.camera, r.config = cam, cfg
+        srv.submit(r)
+    srv.poll()
+    assert [len(l) for l in be.launches] == [2, 2]
+    be.finish(0)
+    srv.poll()
+    assert [len(l) for l in be.launches] == [2, 2, 2]
+
+
+def test_cancel_queued_and_in_flight_requests():
+    """A queued request that is cancelled is never launched; one cancelled
+    while its batch is on the GPU is dropped at collection (its volume is
+    unpinned like the others); neither comes back from poll."""
+    srv, be = make(3, max_batch_requests=2, pipeline=1)
+    a, b, c, d = (srv.submit(req(k)) for k in "ABCA")
+    srv.poll()                                   # A, B launched; C, A queued
+    assert [n for n, _, _ in be.launches[0]] == ["A", "B"]
+    assert srv.cancel(b) and srv.cancel(c)       # in flight, queued
+    assert srv.cancel(c)                         # idempotent
+    assert srv.pending() == 3 and srv.cancelled == 2
+    be.finish(0)
+    done = srv.poll()                            # collects A (B dropped); launches the remaining A
+    assert done == [a] and a.rgba is not None and b.rgba is None and b.done is None
+    assert [n for n, _, _ in be.launches[1]] == ["A"]
+    assert srv._resident["B"].pins == 0
+    be.finish(0)
+    assert srv.poll() == [d] and srv.pending() == 0
+    assert not srv.cancel(a)                     # already completed
+    assert not srv.cancel(req("A"))              # never submitted
