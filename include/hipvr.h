@@ -316,6 +316,33 @@ int hvr_batch_counts(hvr_batch *batch, int index, const uint32_t **counts);
 /* Device time of the last run's graph (ms), without the copy back. */
 int hvr_batch_last_ms(hvr_batch *batch, float *ms);
 
+/* ---- frames without allocation ----
+ * Size of the static device array (one per GPU, held until the process
+ * ends) that hvr_render_async_to_noalloc uses: 4 KiB of request block and
+ * one transparency flag per brick of the bound volume. */
+#define HVR_NOALLOC_MAX_BRICKS (1u << 22)
+#define HVR_NOALLOC_SCRATCH_BYTES (4096u + HVR_NOALLOC_MAX_BRICKS)
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory while rendering (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the frame hvr_render_async_to renders for
+ * the context's current bindings, LUT and parameters -- the same bytes --
+ * into `device_rgba8` (width*height*4 bytes, bottom row first), enqueued on
+ * the context's stream without waiting.  Everything the frame needs beyond
+ * its inputs and output (request block, tile counter, one transparency flag
+ * per brick) lives in a static device array of HVR_NOALLOC_SCRATCH_BYTES that
+ * each GPU holds until the process ends; the call allocates nothing and
+ * calls no allocating routine.  Limits (HVR_EINVAL / HVR_ESTATE with a
+ * message otherwise): the scalar sampler reads an uploaded scalar field of a
+ * volume with brick edge 8 or 16, the normals come from that volume or none,
+ * the LUT has at most 2048 entries, the volume at most HVR_NOALLOC_MAX_BRICKS
+ * bricks (4,194,304: e.g. 1024^3 with 8^3 bricks).  Every call on a GPU
+ * shares that GPU's array, which keeps the last call's contents: calls that
+ * overlap on one GPU (from any context or stream) race; calls on one stream
+ * are ordered.  hvr_last_kernel_ms reports the frame's staging + flags and
+ * march times. */
+int hvr_render_async_to_noalloc(hvr_ctx *ctx, void *device_rgba8);
+
 #ifdef __cplusplus
 }
 #endif

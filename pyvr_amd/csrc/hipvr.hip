@@ -359,6 +359,10 @@ static int enqueue_frame(hvr_ctx *c, bool count, uchar4 *dst)
     return HVR_OK;
 }
 
+// Per-device address of the no-allocation entry point's scratch array,
+// looked up once per device at context creation (defined with the batches).
+static void noalloc_setup(int device);
+
 // ---------------------------------------------------------------------------
 // C ABI
 template <int F, int LB>
@@ -457,6 +461,7 @@ int hvr_create(int device, int width, int height, hvr_ctx **out)
     const char *gather = getenv("HVR_GATHER");
     c->force_global = gather && strcmp(gather, "global") == 0;
     const size_t npix = (size_t)width * height;
+    noalloc_setup(device);
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device);
     if (e == hipSuccess) e = hipMalloc(&c->d_out, npix * sizeof(uchar4));
@@ -1563,9 +1568,126 @@ int hvr_batch_last_ms(hvr_batch *b, float *ms)
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
-// Volume read-back
+// A frame rendered without allocating (hvr_render_async_to_noalloc): the
+// one-request form of the batch launch, whose block (descriptor, tile
+// counter) and per-brick transparency flags live in a static device array
+// instead of buffers the library allocates.  The block is written on the
+// device by k_stage_request from kernel arguments, so the frame is three
+// kernels on the context's stream: stage, brick flags, persistent march.
+
+static constexpr size_t kNoallocFlagsOffset = 4096;
+static_assert(HVR_NOALLOC_SCRATCH_BYTES == kNoallocFlagsOffset + HVR_NOALLOC_MAX_BRICKS, "scratch layout");
+
+// Layout of the scratch array: the batch header at offset 0 (the batch
+// kernels read it at the block's start), the tile counter, the prefix counts
+// of the unbound LUT, the request descriptor, then one flag byte per brick.
+static constexpr StageOffsets kNoallocOffsets = {128u, 192u, 256u};
+static_assert(sizeof(BatchHeader) <= 128, "header fits before the counter");
+static_assert(256 + sizeof(ReqDesc) <= kNoallocFlagsOffset, "descriptor fits before the flags");
+
+static __device__ __align__(256) unsigned char scratch[HVR_NOALLOC_SCRATCH_BYTES];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices];         // device address of `scratch` per device
+static hipError_t g_scratch_err[kMaxDevices];    // the lookup's result per device
+
+// With `device` current: the address of this device's copy of `scratch`
+// (each device holds its own), looked up once.  The array's name is a
+// host-side handle, not a device address; only this pointer is passed to
+// kernels.
+static void noalloc_setup(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_err[device] = e;
+    if (e == hipSuccess) g_scratch_ptr[device] = p;
+}
+
+static int noalloc_scratch(int device, char **out)
+{
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return fail(HVR_EINVAL, "device index out of range for scratch");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        const hipError_t e = g_scratch_err[device];
+        g_err = std::string("no scratch address for this device (hipGetSymbolAddress: ") +
+                (e == hipSuccess ? "not looked up" : hipGetErrorString(e)) + ")";
+        return HVR_EDEVICE;
+    }
+    *out = (char *)g_scratch_ptr[device];
+    return HVR_OK;
+}
 
 extern "C" {
+
+int hvr_render_async_to_noalloc(hvr_ctx *c, void *dst)
+{
+    if (!c || !dst) return fail(HVR_EINVAL, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    const hvr_volume *S = c->scalar_vol, *N = c->normal_vol;
+    if (!S || c->scalar_role != HVR_ROLE_SCALAR || !fmt_has_scalar(S->fmt) || !S->scalar_loaded || !S->d_range)
+        return fail(HVR_ESTATE, "hvr_render_async_to_noalloc: the scalar sampler must read an uploaded scalar field");
+    int nm;
+    if (!N) nm = NM_ZERO;
+    else if (N == S && c->normal_role == HVR_ROLE_SCALAR) nm = NM_ALIAS;
+    else if (N == S && c->normal_role == HVR_ROLE_NORMAL && fmt_has_normal(S->fmt)) nm = NM_SAME;
+    else return fail(HVR_ESTATE, "hvr_render_async_to_noalloc: normals must come from the scalar volume, or none");
+    const bool buf = volume_buf(S) && !c->force_global;
+    const BatchLaunchFn fn = batch_fn(S->fmt, nm, S->lb, false, HVR_TILE_8X8, buf);
+    if (!fn) return fail(HVR_EINVAL, "hvr_render_async_to_noalloc: brick edge must be 8 or 16 (log2 3 or 4)");
+    if (c->lut_n > kLutLdsMax) return fail(HVR_EINVAL, "hvr_render_async_to_noalloc: LUT above 2048 entries");
+    const uint64_t nb = (uint64_t)S->nbx * S->nby * S->nbz;
+    if (nb > HVR_NOALLOC_MAX_BRICKS)
+        return fail(HVR_EINVAL, "hvr_render_async_to_noalloc: volume has more bricks than the scratch array's "
+                                "flags hold (HVR_NOALLOC_MAX_BRICKS)");
+    char *blk = nullptr;
+    if (const int rc = noalloc_scratch(c->device, &blk)) return rc;
+
+    const StageOffsets off = kNoallocOffsets;
+    BatchHeader hdr;
+    memset(&hdr, 0, sizeof(hdr));
+    hdr.nclass = 1;
+    hdr.nreq = 1;
+    hdr.all_offset = off.desc;
+    hdr.cls[0].desc_offset = off.desc;
+    hdr.cls[0].nreq = 1;
+    // the block tile of k_batch with 8x8 ray tiles: 16 x (4 * kBatchWaves / 2)
+    const int bx = 16, by = 8 * (kBatchWaves / 2);
+    ReqDesc d;
+    memset(&d, 0, sizeof(d));
+    d.fc = frame_const(c);
+    d.ts = tex_desc(S, HVR_ROLE_SCALAR, d.fc);
+    d.lut = c->d_lut;
+    d.nz = c->d_nz ? (const int *)c->d_nz : (const int *)(blk + off.unbound_nz);
+    d.range = S->d_range;
+    d.flags = (uint8_t *)(blk + kNoallocFlagsOffset);
+    d.out = (uchar4 *)dst;
+    d.counts = nullptr;
+    d.nbricks = (uint32_t)nb;
+    d.lut_n = c->lut_n;
+    d.ntx = (c->W + bx - 1) / bx;
+    d.tile_end = d.ntx * ((c->H + by - 1) / by);
+
+    hipEvent_t e0 = c->ev[0], e1 = c->ev[1], e2 = c->ev[2];
+    HIPCHK(hipEventRecord(e0, c->stream));
+    hipLaunchKernelGGL(k_stage_request, dim3(1), dim3(64), 0, c->stream, blk, off, hdr, d);
+    hipLaunchKernelGGL(k_batch_flags, dim3(1024), dim3(256), 0, c->stream, (const char *)blk);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(e1, c->stream));
+    if (const int rc = fn(blk, 0, (uint32_t *)(blk + off.counter), (int)align_up((size_t)c->lut_n, 256), c->ncu,
+                          c->stream))
+        return fail(rc, "hvr_render_async_to_noalloc: march launch failed");
+    HIPCHK(hipEventRecord(e2, c->stream));
+    c->last_ev[0] = e0;
+    c->last_ev[1] = e1;
+    c->last_ev[2] = e2;
+    c->events_recorded = true;
+    return HVR_OK;
+}
 
 int hvr_volume_download(hvr_volume *v, float *scalar, float *normals)
 {

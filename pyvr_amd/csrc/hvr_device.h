@@ -1343,6 +1343,31 @@ __global__ void __launch_bounds__(64) k_fill_u32(uint32_t *__restrict__ p, int n
     for (int i = threadIdx.x; i < n; i += 64) p[i] = v;
 }
 
+// A one-request batch block written on the device from kernel arguments (so
+// that a frame needs no host-to-device copy and no staging allocation): the
+// header at the start of the block (where k_batch and k_batch_flags read
+// it), the tile counter reset to 0, the prefix counts of the unbound LUT
+// (0, 1), and the descriptor, at the offsets the host chose.
+struct StageOffsets {
+    uint32_t counter, unbound_nz, desc;
+};
+
+__global__ void __launch_bounds__(64)
+k_stage_request(char *__restrict__ blk, const StageOffsets off, const BatchHeader hdr, const ReqDesc desc)
+{
+    const int lane = threadIdx.x;
+    if (lane == 0) {
+        *(uint32_t *)(blk + off.counter) = 0u;
+        ((int *)(blk + off.unbound_nz))[0] = 0;
+        ((int *)(blk + off.unbound_nz))[1] = 1;
+    }
+    static_assert(sizeof(BatchHeader) % 4 == 0 && sizeof(ReqDesc) % 4 == 0, "dword copies");
+    const uint32_t *h = (const uint32_t *)&hdr, *d = (const uint32_t *)&desc;
+    uint32_t *hh = (uint32_t *)blk, *dd = (uint32_t *)(blk + off.desc);
+    for (int i = lane; i < (int)(sizeof(BatchHeader) / 4); i += 64) hh[i] = h[i];
+    for (int i = lane; i < (int)(sizeof(ReqDesc) / 4); i += 64) dd[i] = d[i];
+}
+
 // Decoded texels of a volume in numpy C order (texel (x, y, z) at flat
 // x + s0*(y + s1*z)), z-planes z0 .. z0 + nz - 1: the scalar as the march
 // reads it (UNORM8 times 1/255) and the normal as stored (SNORM8 as its

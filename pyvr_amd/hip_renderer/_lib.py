@@ -42,7 +42,7 @@ EXPORTS = (
     "hvr_batch_add", "hvr_batch_run_async", "hvr_batch_wait", "hvr_batch_run", "hvr_batch_output",
     "hvr_batch_counts", "hvr_batch_last_ms", "hvr_volume_upload_scalar_normals", "hvr_volume_generate_blobs",
     "hvr_batch_query", "hvr_device_memory", "hvr_read_pinned", "hvr_volume_download",
-    "hvr_resize", "hvr_debug_inject_fault",
+    "hvr_resize", "hvr_debug_inject_fault", "hvr_render_async_to_noalloc",
 )
 
 # batch ray-tile shapes (hvr_batch_set_tile_shape)
@@ -126,6 +126,7 @@ def _declare(L):
         "hvr_volume_generate_blobs": ([vp, i, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(ctypes.c_double), ctypes.c_double, ctypes.c_ulonglong], i),
         "hvr_render_async_to": ([vp, vp], i),
+        "hvr_render_async_to_noalloc": ([vp, vp], i),
         "hvr_device_alloc": ([vp, ctypes.c_uint64, ctypes.POINTER(vp)], i),
         "hvr_device_free": ([vp, vp], i),
         "hvr_copy_to_host": ([vp, vp, vp, ctypes.c_uint64], i),
