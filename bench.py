@@ -288,12 +288,15 @@ def plain_leg(wl: Workload, args, sf, nf):
     for b in bufs:
         L.hvr_device_free(ctx, b)
 
-    # end-to-end frames: camera update, draw, RGBA8 copy to the host
+    # end-to-end frames as VolumeRenderer.set_camera + render() make them:
+    # camera update, draw, RGBA8 copy into a new bytes object
     lat = []
+    pr.prepare(wl.volume, wl.cams[0], wl.ctf, wl.otf, wl.width, wl.height, wl.config, wl.light, wl.tau)
     for i in range(args.latency_frames + 2):
         cam = wl.cams[i % len(wl.cams)]
         t = time.perf_counter()
-        pr.render(wl.volume, cam, wl.ctf, wl.otf, wl.width, wl.height, wl.config, wl.light, wl.tau)
+        pr.set_camera(cam)
+        pr.read()
         if i >= 2:
             lat.append((time.perf_counter() - t) * 1e3)
     lat = np.array(lat)
@@ -324,12 +327,12 @@ def batched_leg(wl: Workload, args, sf, nf, s_step):
         t0 = time.perf_counter()
         bricks = (args.brick_log2,) if args.brick_log2 >= 0 else (3, 4)
         tuning = tune(vols, reqs, sf, nf, bricks=bricks)
-        lb, tile = tuning["brick_log2"], tuning["tile_shape"]
-        log(f"autotuner ({time.perf_counter() - t0:.1f}s): " +
-            ", ".join(f"lb{a}/{t} {ms:.2f} ms" for a, t, ms in tuning["table"]) + f" -> lb{lb}/{tile}")
+        lb, tile, kernel = tuning["brick_log2"], tuning["tile_shape"], tuning["kernel"]
+        log(f"autotuner ({time.perf_counter() - t0:.1f}s, {len(tuning['table'])} candidates): " +
+            ", ".join(f"lb{a}/{t}/{k} {ms:.2f}" for a, t, k, ms in tuning["table"]) + f" ms -> lb{lb}/{tile}/{kernel}")
     else:
-        lb, tile = (args.brick_log2 if args.brick_log2 >= 0 else 3), args.tile_shape
-    b = BatchRenderer(scalar_format=sf, normal_format=nf, brick_log2=lb, tile_shape=tile, pipeline=2)
+        lb, tile, kernel = (args.brick_log2 if args.brick_log2 >= 0 else 3), args.tile_shape, args.kernel
+    b = BatchRenderer(scalar_format=sf, normal_format=nf, brick_log2=lb, tile_shape=tile, pipeline=2, kernel=kernel)
     try:
         for q in reqs:
             b.add(vols[0], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
@@ -371,8 +374,9 @@ def batched_leg(wl: Workload, args, sf, nf, s_step):
             "frames_per_s": len(reqs) * args.steps / wall, "device_ms_per_run": dev,
             "batch_ms_p50": float(np.percentile(lat, 50)), "batch_ms_p99": float(np.percentile(lat, 99)),
             "brick_log2": lb, "tile_shape": tile,
-            "autotune": None if tuning is None else [{"brick_log2": a, "tile_shape": t, "device_ms": ms}
-                                                     for a, t, ms in tuning["table"]]}
+            "kernel": kernel,
+            "autotune": None if tuning is None else [{"brick_log2": a, "tile_shape": t, "kernel": k, "device_ms": ms}
+                                                     for a, t, k, ms in tuning["table"]]}
 
 
 def serving_leg(args, rates, duration, nvol, plain_requests):
@@ -398,7 +402,7 @@ def serving_leg(args, rates, duration, nvol, plain_requests):
     reqs = scenes.c4_requests(n_max, nvol=nvol)
     srv = RenderServer(scalar_format=sf, normal_format=nf, brick_log2=lb, tile_shape=tile,
                        max_batch_requests=args.serve_max_requests, max_batch_pixels=1 << 26, pipeline=3,
-                       keep_results=False, max_batch_samples=args.serve_max_samples)
+                       keep_results=False, max_batch_samples=args.serve_max_samples or None)
     t0 = time.perf_counter()
     for k, v in enumerate(vols):
         srv.register(k, v)
@@ -455,16 +459,22 @@ def serving_leg(args, rates, duration, nvol, plain_requests):
             f"{row['batches']} batches")
     srv.close()
 
-    # plain: one request per launch, synchronously, same requests and volumes
+    # plain: one request per launch, synchronously, same requests and
+    # volumes (all resident before timing), each image copied to pinned host
+    # memory as a served one is
     pr = PlainRenderer(scalar_format=sf, normal_format=nf, brick_log2=lb)
     n_plain = min(len(reqs), plain_requests)
     lat = []
     try:
-        for q in reqs[:min(8, n_plain)]:      # warm-up (volumes resident, kernels loaded)
-            pr.render(vols[q.volume], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
+        for v in vols:
+            pr.upload(v)
+        for q in reqs[:min(8, n_plain)]:      # warm-up (kernels loaded, buffers at size)
+            pr.prepare(vols[q.volume], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
+            pr.read_pinned()
         for q in reqs[:n_plain]:
             t = time.perf_counter()
-            pr.render(vols[q.volume], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
+            pr.prepare(vols[q.volume], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
+            pr.read_pinned()
             lat.append(time.perf_counter() - t)
     finally:
         pr.close()
@@ -472,7 +482,7 @@ def serving_leg(args, rates, duration, nvol, plain_requests):
     plain = {"requests": n_plain, "p50_ms": float(np.percentile(lat, 50) * 1e3),
              "p99_ms": float(np.percentile(lat, 99) * 1e3),
              "samples_per_s": float(samples[:n_plain].sum() / lat.sum()),
-             "path": "PlainRenderer.render: single-frame kernels, one request per launch, image read back"}
+             "path": "PlainRenderer: single-frame kernels, one request per launch, image to pinned host memory"}
     log(f"serving plain: p50 {plain['p50_ms']:.2f} ms, p99 {plain['p99_ms']:.2f} ms, "
         f"{plain['samples_per_s'] / 1e9:.1f} G samples/s")
     best = max(table, key=lambda r: r["samples_per_s"])
@@ -509,13 +519,13 @@ def bench_batch(args):
         t0 = time.perf_counter()
         bricks = (args.brick_log2,) if args.brick_log2 >= 0 else (3, 4)
         tuning = tune(vols, reqs, sf, nf, bricks=bricks)
-        lb, tile = tuning["brick_log2"], tuning["tile_shape"]
-        log(f"autotuner ({time.perf_counter() - t0:.1f}s): " +
-            ", ".join(f"lb{a}/{t} {ms:.2f} ms" for a, t, ms in tuning["table"]) + f" -> lb{lb}/{tile}")
+        lb, tile, kernel = tuning["brick_log2"], tuning["tile_shape"], tuning["kernel"]
+        log(f"autotuner ({time.perf_counter() - t0:.1f}s, {len(tuning['table'])} candidates): " +
+            ", ".join(f"lb{a}/{t}/{k} {ms:.2f}" for a, t, k, ms in tuning["table"]) + f" ms -> lb{lb}/{tile}/{kernel}")
     else:
-        lb, tile = (args.brick_log2 if args.brick_log2 >= 0 else 3), args.tile_shape
+        lb, tile, kernel = (args.brick_log2 if args.brick_log2 >= 0 else 3), args.tile_shape, args.kernel
 
-    b = BatchRenderer(scalar_format=sf, normal_format=nf, brick_log2=lb, tile_shape=tile, pipeline=2)
+    b = BatchRenderer(scalar_format=sf, normal_format=nf, brick_log2=lb, tile_shape=tile, pipeline=2, kernel=kernel)
     for q in reqs:
         b.add(vols[q.volume], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
     b.run(count=True)
@@ -592,9 +602,9 @@ def bench_batch(args):
                                f"96x160x224, images 256^2-1024^2), one persistent graph launch per run",
                    "requests": len(reqs), "pixels_per_step": pixels, "samples_per_step": samples,
                    "scalar_format": sf, "normal_format": nf, "brick_log2": lb, "tile_shape": tile,
-                   "autotuned": tuning is not None},
-        "autotune": None if tuning is None else [{"brick_log2": a, "tile_shape": t, "device_ms": ms}
-                                                 for a, t, ms in tuning["table"]],
+                   "kernel": kernel, "autotuned": tuning is not None},
+        "autotune": None if tuning is None else [{"brick_log2": a, "tile_shape": t, "kernel": k, "device_ms": ms}
+                                                 for a, t, k, ms in tuning["table"]],
         "batch_device_ms": float(np.mean(dev_ms)), "batched_one_at_a_time": value_sync,
         "plain_kernel": res["plain_kernel"], "plain_api": res["plain_api"],
         "batched_vs_plain_kernel": value / res["plain_kernel"],
@@ -633,6 +643,8 @@ def main():
     ap.add_argument("--requests", type=int, default=64, help="c3: requests per batch")
     ap.add_argument("--tile-shape", default="auto", choices=["auto", "8x8", "16x4"],
                     help="ray tile of a wavefront (auto: the autotuner picks it and the brick size)")
+    ap.add_argument("--kernel", default="default", help="batched kernel variant when --tile-shape is not auto "
+                    "(default, waves2, waves8, nogate, noskip)")
     ap.add_argument("--size", type=int, default=0)
     ap.add_argument("--image", type=int, default=0)
     ap.add_argument("--cameras", type=int, default=24)
@@ -643,8 +655,8 @@ def main():
     ap.add_argument("--serve-rates", default="500,1000", help="serving leg of the default run (req/s)")
     ap.add_argument("--serve-duration", type=float, default=1.0, help="serving leg: seconds of arrivals per rate")
     ap.add_argument("--serve-max-requests", type=int, default=64)
-    ap.add_argument("--serve-max-samples", type=float, default=None,
-                    help="serving: bound a batch's estimated samples (None: requests and pixels only)")
+    ap.add_argument("--serve-max-samples", type=float, default=2e9,
+                    help="serving: bound on a batch's estimated samples (0: requests and pixels only)")
     ap.add_argument("--no-serve", action="store_true", help="default run without the serving leg")
     ap.add_argument("--brick-log2", type=int, default=-1)
     ap.add_argument("--skip", choices=("auto", "on", "off"), default="auto",
@@ -678,7 +690,8 @@ def main():
     roofline = plain["roofline"]
     if args.workload == "c2" and not args.plain_only:
         batched = batched_leg(wl, args, sf, nf, s_step)
-        bkey = f"{wl.name}/{len(wl.cams)}cams/{sf}+{nf}/batch/lb{batched['brick_log2']}/{batched['tile_shape']}"
+        bkey = (f"{wl.name}/{len(wl.cams)}cams/{sf}+{nf}/batch/lb{batched['brick_log2']}/{batched['tile_shape']}"
+                f"/{batched['kernel']}")
         roofline = roofline_of("hvr::k_batch (graph launch: staging copy, counter fill, brick flags, k_batch)",
                                batched["samples"], bpv, 4 * npix * len(wl.cams), batched["device_ms_per_run"], bkey)
     serving = None
@@ -696,7 +709,8 @@ def main():
             "fps": batched["frames_per_s"],
             "batch_ms_p50": batched["batch_ms_p50"], "batch_ms_p99": batched["batch_ms_p99"],
             "frame_ms_p50": plain["frame_ms_p50"], "frame_ms_p99": plain["frame_ms_p99"],
-            "batched": {k: batched[k] for k in ("device_ms_per_run", "brick_log2", "tile_shape", "autotune")},
+            "batched": {k: batched[k] for k in ("device_ms_per_run", "brick_log2", "tile_shape", "kernel",
+                                                "autotune")},
             "plain_kernel": plain_out,
             "batched_vs_plain_kernel": batched["value"] / plain["value"],
         }

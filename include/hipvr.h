@@ -294,6 +294,12 @@ int hvr_batch_create(hvr_ctx *ctx, hvr_batch **out);
 int hvr_batch_destroy(hvr_batch *batch);
 int hvr_batch_clear(hvr_batch *batch);
 int hvr_batch_set_tile_shape(hvr_batch *batch, int shape);
+/* Kernel variant of the batch's non-counting runs (counting runs use the
+ * default): waves per block and whether transparent bricks are stepped over
+ * (skip) and normals interpolated only for samples of non-zero alpha (gate).
+ * Supported: (4, 1, 1) -- the default --, (2, 1, 1), (8, 1, 1), (4, 1, 0),
+ * (4, 0, 1).  Every variant gives the same image; the autotuner times them. */
+int hvr_batch_set_kernel(hvr_batch *batch, int waves_per_block, int skip, int gate);
 /* Add a request; lut_rgba holds lut_n RGBA float entries (lut_n = 0: no
  * LUT bound, reads (0, 0, 0, 1)); *index receives the request's index. */
 int hvr_batch_add(hvr_batch *batch, hvr_volume *scalar_vol, int scalar_role, hvr_volume *normal_vol,

@@ -2,7 +2,7 @@
 // of hvr_device.h.  Host side: contexts (framebuffer, rays, LUT, stream,
 // events), volumes (device texels in brick layout), FP64 uniform
 // preparation and kernel dispatch.
-#include "hvr_device.h"
+#include "hvr_launch.h"
 #include "../../include/hipvr.h"
 
 #include <math.h>
@@ -105,6 +105,7 @@ struct hvr_ctx {
     int timing_max, timing_n;
     // transparent-brick skipping
     int *d_nz;             // prefix counts of non-transparent LUT entries (lut_n + 1)
+    int nz_cap;            // entries d_nz holds counts for
     uint32_t lut_gen;
     uint8_t *d_flags;      // per-brick transparent flags of flags_vol
     uint32_t *d_nclear;    // transparent-brick count written by k_brick_flags
@@ -206,14 +207,13 @@ static TexDesc tex_desc(const hvr_volume *v, int role, const FrameConst &fc)
 
 // ---------------------------------------------------------------------------
 // kernel dispatch
-template <int SF, int NM, int LB, bool COUNT, bool SKIP = false, bool BUF = false>
-static void launch_march_t(hvr_ctx *c, const FrameConst &fc, const TexDesc &ts, const TexDesc &tn,
-                           int ntx, int nblocks, const uint8_t *skip, uchar4 *out)
+// The generic march instances (runtime format; no scalar field bound).
+template <int SF, int NM, bool COUNT>
+static void launch_march_generic(const MarchArgs &a)
 {
-    const size_t lds = NM == NM_ANY ? 0 : 2 * (size_t)c->lut_n * sizeof(float4);
-    hipLaunchKernelGGL((k_march<SF, NM, LB, COUNT, SKIP, BUF>), dim3(nblocks), dim3(256), lds, c->stream, fc,
-                       ts, tn, (const float4 *)c->d_lut, c->lut_n, (const RayRec *)c->d_rays, out,
-                       c->d_counts, skip, ntx, nblocks);
+    const size_t lds = NM == NM_ANY ? 0 : 2 * (size_t)a.lut_n * sizeof(float4);
+    hipLaunchKernelGGL((k_march<SF, NM, 3, COUNT, false, false>), dim3(a.nblocks), dim3(256), lds, a.stream, a.fc,
+                       a.ts, a.tn, a.lut, a.lut_n, a.rays, a.out, a.counts, a.skip, a.ntx, a.nblocks);
 }
 
 // Transparent-brick flags of volume v for the current LUT, recomputed (and
@@ -256,78 +256,65 @@ static const uint8_t *brick_flags(hvr_ctx *c, const hvr_volume *v)
     return nullptr;
 }
 
-template <int SF, int LB, bool COUNT, bool SKIP, bool BUF>
-static void launch_fast_nm(hvr_ctx *c, const FrameConst &fc, const TexDesc &ts, int nm, int ntx, int nblocks,
-                           const uint8_t *skip, uchar4 *out)
+// The march of one frame into `out` (W*H RGBA8 on the device): the fast
+// instances of the scalar volume's format (normals from the same texels, as
+// (rho, 0, 0) of the same volume, or unbound; brick edge 8 or 16), with
+// transparent-brick skipping when the flags say so and buffer loads below
+// 4 GiB, else the generic instance.
+static void launch_march(hvr_ctx *c, const FrameConst &fc, uchar4 *out, bool count)
 {
-    const TexDesc none = tex_desc(nullptr, 0, fc);
-    if (nm == NM_ZERO) launch_march_t<SF, NM_ZERO, LB, COUNT, SKIP, BUF>(c, fc, ts, none, ntx, nblocks, skip, out);
-    else if (nm == NM_ALIAS)
-        launch_march_t<SF, NM_ALIAS, LB, COUNT, SKIP, BUF>(c, fc, ts, none, ntx, nblocks, skip, out);
-    else if constexpr (fmt_has_normal(SF))
-        launch_march_t<SF, NM_SAME, LB, COUNT, SKIP, BUF>(c, fc, ts, none, ntx, nblocks, skip, out);
-}
-
-// Fast-path instance: transparent-brick skipping when the flags say so,
-// buffer loads when the volume's allocation is below 4 GiB.
-template <int SF, int LB, bool COUNT>
-static void launch_fast(hvr_ctx *c, const FrameConst &fc, const TexDesc &ts, const hvr_volume *S, int nm,
-                        int ntx, int nblocks, uchar4 *out)
-{
-    const uint8_t *skip = brick_flags(c, S);
-    c->skip_used = skip != nullptr;
-    const bool buf = ts.nbytes32 != 0 && !c->force_global;
-    if (skip && buf) launch_fast_nm<SF, LB, COUNT, true, true>(c, fc, ts, nm, ntx, nblocks, skip, out);
-    else if (skip) launch_fast_nm<SF, LB, COUNT, true, false>(c, fc, ts, nm, ntx, nblocks, skip, out);
-    else if (buf) launch_fast_nm<SF, LB, COUNT, false, true>(c, fc, ts, nm, ntx, nblocks, nullptr, out);
-    else launch_fast_nm<SF, LB, COUNT, false, false>(c, fc, ts, nm, ntx, nblocks, nullptr, out);
-}
-
-// The march of one frame into `out` (W*H RGBA8 on the device).
-template <bool COUNT>
-static void launch_march(hvr_ctx *c, const FrameConst &fc, uchar4 *out)
-{
-    const int ntx = (c->W + 15) / 16, nty = (c->H + 15) / 16;
-    const int nblocks = ntx * nty;
+    MarchArgs a;
+    a.fc = fc;
+    a.ntx = (c->W + 15) / 16;
+    a.nblocks = a.ntx * ((c->H + 15) / 16);
     const hvr_volume *S = c->scalar_vol, *N = c->normal_vol;
-    const TexDesc ts = tex_desc(S, c->scalar_role, fc), tn = tex_desc(N, c->normal_role, fc);
-    const TexDesc none = tex_desc(nullptr, 0, fc);
+    a.ts = tex_desc(S, c->scalar_role, fc);
+    a.tn = tex_desc(N, c->normal_role, fc);
+    a.lut = (const float4 *)c->d_lut;
+    a.lut_n = c->lut_n;
+    a.rays = (const RayRec *)c->d_rays;
+    a.out = out;
+    a.counts = c->d_counts;
+    a.skip = nullptr;
+    a.stream = c->stream;
     const bool small_lut = c->lut_n <= kLutLdsMax;
     c->skip_used = 0;
     if (small_lut && !S && !N) {
-        launch_march_t<TF_NONE, NM_ZERO, 3, COUNT>(c, fc, none, none, ntx, nblocks, nullptr, out);
+        a.tn = a.ts = tex_desc(nullptr, 0, fc);
+        if (count) launch_march_generic<TF_NONE, NM_ZERO, true>(a);
+        else launch_march_generic<TF_NONE, NM_ZERO, false>(a);
         return;
     }
-    // Fast paths: scalar field of one volume, normals from the same texels,
-    // (rho, 0, 0) of the same volume, or unbound; brick edge 8 or 16.
     if (small_lut && S && c->scalar_role == HVR_ROLE_SCALAR && (S->lb == 3 || S->lb == 4)) {
         int nm = -1;
         if (!N) nm = NM_ZERO;
         else if (N == S && c->normal_role == HVR_ROLE_SCALAR) nm = NM_ALIAS;
         else if (N == S && c->normal_role == HVR_ROLE_NORMAL && fmt_has_scalar(S->fmt)) nm = NM_SAME;
-#define HVR_CASE(SF)                                                                          \
-    case SF:                                                                                  \
-        if (S->lb == 3) launch_fast<SF, 3, COUNT>(c, fc, ts, S, nm, ntx, nblocks, out);       \
-        else launch_fast<SF, 4, COUNT>(c, fc, ts, S, nm, ntx, nblocks, out);                  \
-        return;
         if (nm >= 0) {
+            const uint8_t *skip = brick_flags(c, S);
+            const bool buf = a.ts.nbytes32 != 0 && !c->force_global;
+            a.skip = skip;
+            a.tn = tex_desc(nullptr, 0, fc);
+            bool done = false;
             switch (S->fmt) {
-                HVR_CASE(TF_S_F32)
-                HVR_CASE(TF_S_F16)
-                HVR_CASE(TF_S_U8)
-                HVR_CASE(TF_C_F32_F32)
-                HVR_CASE(TF_C_F16_F32)
-                HVR_CASE(TF_C_U8_F32)
-                HVR_CASE(TF_C_F32_SN8)
-                HVR_CASE(TF_C_F16_SN8)
-                HVR_CASE(TF_C_U8_SN8)
-                HVR_CASE(TF_C_F16_F16)
+#define HVR_CASE(F)                                                            \
+    case F:                                                                    \
+        done = launch_march_##F(a, nm, S->lb, count, skip != nullptr, buf);    \
+        break;
+                HVR_FAST_FORMATS(HVR_CASE)
+#undef HVR_CASE
             default: break;
             }
+            if (done) {
+                c->skip_used = skip != nullptr;
+                return;
+            }
+            a.skip = nullptr;
+            a.tn = tex_desc(N, c->normal_role, fc);
         }
-#undef HVR_CASE
     }
-    launch_march_t<TF_NONE, NM_ANY, 3, COUNT>(c, fc, ts, tn, ntx, nblocks, nullptr, out);
+    if (count) launch_march_generic<TF_NONE, NM_ANY, true>(a);
+    else launch_march_generic<TF_NONE, NM_ANY, false>(a);
 }
 
 static int enqueue_frame(hvr_ctx *c, bool count, uchar4 *dst)
@@ -347,9 +334,7 @@ static int enqueue_frame(hvr_ctx *c, bool count, uchar4 *dst)
     hipLaunchKernelGGL(k_ray_setup, dim3((npix + 255) / 256), dim3(256), 0, c->stream, fc, c->d_rays);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(e1, c->stream));
-    uchar4 *out = dst ? dst : c->d_out;
-    if (count) launch_march<true>(c, fc, out);
-    else launch_march<false>(c, fc, out);
+    launch_march(c, fc, dst ? dst : c->d_out, count);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(e2, c->stream));
     c->last_ev[0] = e0;
@@ -869,9 +854,13 @@ int hvr_set_lut(hvr_ctx *c, const float *rgba, int n)
     HIPCHK(hipMemcpy(c->d_lut, src, (size_t)cnt * sizeof(float4), hipMemcpyHostToDevice));
     std::vector<int> nz((size_t)cnt + 1, 0);
     lut_prefix_counts(src, cnt, nz.data());
-    (void)hipFree(c->d_nz);
-    c->d_nz = nullptr;
-    HIPCHK(hipMalloc(&c->d_nz, nz.size() * sizeof(int)));
+    if (cnt > c->nz_cap) {
+        (void)hipFree(c->d_nz);
+        c->d_nz = nullptr;
+        c->nz_cap = 0;
+        HIPCHK(hipMalloc(&c->d_nz, nz.size() * sizeof(int)));
+        c->nz_cap = cnt;
+    }
     HIPCHK(hipMemcpy(c->d_nz, nz.data(), nz.size() * sizeof(int), hipMemcpyHostToDevice));
     c->lut_n = cnt;
     ++c->lut_gen;
@@ -1130,6 +1119,7 @@ struct hvr_batch {
     hipStream_t stream;
     std::vector<BatchReq> reqs;
     int tile_shape;                       // HVR_TILE_8X8 / HVR_TILE_16X4
+    int variant;                          // kernel variant of non-counting runs (BV_*)
     char *h_stage, *d_stage;              // staged block (pinned host, device)
     size_t stage_cap;
     uchar4 *d_out;                        // output pool, one RGBA8 image per request
@@ -1154,69 +1144,19 @@ struct hvr_batch {
 // descriptor (pyvr_amd/hip_renderer/serve.py, batch_pool_bytes).
 static_assert(sizeof(ReqDesc) <= 512, "request descriptor above the serving budget's bound");
 
-typedef int (*BatchLaunchFn)(const char *blk, int cls, uint32_t *ctr, int lut_cap, int ncu, hipStream_t st);
-
-template <int SF, int NM, int LB, bool COUNT, int TS, bool BUF>
-static int batch_launch(const char *blk, int cls, uint32_t *ctr, int lut_cap, int ncu, hipStream_t st)
+// The batched launcher for a class: format, normal mode, brick log2, counting,
+// ray tile, buffer loads and kernel variant (BV_*); nullptr when the library
+// has no such instance.  buf: the volume is read through buffer loads
+// (below 4 GiB and 2^24 bricks).
+static BatchLaunchFn batch_fn(int fmt, int nm, int lb, bool count, int ts, bool buf, int variant = BV_DEFAULT)
 {
-    const size_t lds = 2 * (size_t)lut_cap * sizeof(float4);
-    int per = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch<SF, NM, LB, COUNT, TS, BUF>, 64 * kBatchWaves,
-                                                     lds) !=
-            hipSuccess ||
-        per < 1)
-        per = 1;
-    hipLaunchKernelGGL((k_batch<SF, NM, LB, COUNT, TS, BUF>), dim3(per * ncu), dim3(64 * kBatchWaves), lds, st, blk,
-                       cls, ctr);
-    return hipGetLastError() == hipSuccess ? HVR_OK : HVR_EDEVICE;
-}
-
-template <int SF, int NM, int LB, bool COUNT, bool BUF>
-static BatchLaunchFn batch_pick_ts(int ts)
-{
-    return ts == HVR_TILE_16X4 ? batch_launch<SF, NM, LB, COUNT, 1, BUF> : batch_launch<SF, NM, LB, COUNT, 0, BUF>;
-}
-
-template <int SF, int NM, bool COUNT, bool BUF>
-static BatchLaunchFn batch_pick_lb(int lb, int ts)
-{
-    return lb == 4 ? batch_pick_ts<SF, NM, 4, COUNT, BUF>(ts) : batch_pick_ts<SF, NM, 3, COUNT, BUF>(ts);
-}
-
-template <bool COUNT, bool BUF>
-static BatchLaunchFn batch_fn_c(int fmt, int nm, int lb, int ts)
-{
-#define HVR_S(F)                                                                   \
-    case F:                                                                        \
-        if (nm == NM_ZERO) return batch_pick_lb<F, NM_ZERO, COUNT, BUF>(lb, ts);   \
-        if (nm == NM_ALIAS) return batch_pick_lb<F, NM_ALIAS, COUNT, BUF>(lb, ts); \
-        return nullptr;
-#define HVR_C(F)                                                                   \
-    case F:                                                                        \
-        if (nm == NM_SAME) return batch_pick_lb<F, NM_SAME, COUNT, BUF>(lb, ts);   \
-        return nullptr;
     switch (fmt) {
-        HVR_S(TF_S_F32)
-        HVR_S(TF_S_F16)
-        HVR_S(TF_S_U8)
-        HVR_C(TF_C_F32_F32)
-        HVR_C(TF_C_F16_F32)
-        HVR_C(TF_C_U8_F32)
-        HVR_C(TF_C_F32_SN8)
-        HVR_C(TF_C_F16_SN8)
-        HVR_C(TF_C_U8_SN8)
-        HVR_C(TF_C_F16_F16)
+#define HVR_CASE(F) \
+    case F: return batch_fn_##F(nm, lb, count, ts, buf, variant);
+        HVR_FAST_FORMATS(HVR_CASE)
+#undef HVR_CASE
     default: return nullptr;
     }
-#undef HVR_S
-#undef HVR_C
-}
-
-// buf: the volume is read through buffer loads (below 4 GiB and 2^24 bricks)
-static BatchLaunchFn batch_fn(int fmt, int nm, int lb, bool count, int ts, bool buf)
-{
-    if (buf) return count ? batch_fn_c<true, true>(fmt, nm, lb, ts) : batch_fn_c<false, true>(fmt, nm, lb, ts);
-    return count ? batch_fn_c<true, false>(fmt, nm, lb, ts) : batch_fn_c<false, false>(fmt, nm, lb, ts);
 }
 
 static bool volume_buf(const hvr_volume *v)
@@ -1267,6 +1207,7 @@ int hvr_batch_create(hvr_ctx *c, hvr_batch **out)
     hvr_batch *b = new hvr_batch();
     b->ctx = c;
     b->tile_shape = HVR_TILE_8X8;
+    b->variant = BV_DEFAULT;
     b->exec = nullptr;
     b->graph = nullptr;
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
@@ -1318,6 +1259,22 @@ int hvr_batch_set_tile_shape(hvr_batch *b, int shape)
     if (!b) return fail(HVR_EINVAL, "batch is null");
     if (shape != HVR_TILE_8X8 && shape != HVR_TILE_16X4) return fail(HVR_EINVAL, "unknown tile shape");
     b->tile_shape = shape;
+    return HVR_OK;
+}
+
+int hvr_batch_set_kernel(hvr_batch *b, int waves_per_block, int skip, int gate)
+{
+    if (!b) return fail(HVR_EINVAL, "batch is null");
+    int v = -1;
+    if (waves_per_block == 4 && skip && gate) v = BV_DEFAULT;
+    else if (waves_per_block == 2 && skip && gate) v = BV_WAVES2;
+    else if (waves_per_block == 8 && skip && gate) v = BV_WAVES8;
+    else if (waves_per_block == 4 && skip && !gate) v = BV_NOGATE;
+    else if (waves_per_block == 4 && !skip && gate) v = BV_NOSKIP;
+    if (v < 0)
+        return fail(HVR_EINVAL, "kernel variants: 4 waves with skipping and gating, or one of 2 / 8 waves, "
+                                "no gating, no skipping");
+    b->variant = v;
     return HVR_OK;
 }
 
@@ -1397,9 +1354,11 @@ int hvr_batch_run_async(hvr_batch *b, int count)
             return (long long)b->reqs[a].W * b->reqs[a].H > (long long)b->reqs[z].W * b->reqs[z].H;
         });
     // sizes
-    // the block tile of k_batch: 2 waves across, kBatchWaves / 2 down
+    // the block tile of k_batch: 2 waves across, W / 2 down (counting runs
+    // use the default variant)
+    const int variant = count ? (int)BV_DEFAULT : b->variant;
     const int bx = b->tile_shape == HVR_TILE_16X4 ? 32 : 16;
-    const int by = (b->tile_shape == HVR_TILE_16X4 ? 4 : 8) * (kBatchWaves / 2);
+    const int by = (b->tile_shape == HVR_TILE_16X4 ? 4 : 8) * (batch_variant_waves(variant) / 2);
     size_t desc_off = align_up(sizeof(BatchHeader), 256);
     size_t lut_off = align_up(desc_off + (size_t)nreq * sizeof(ReqDesc), 256);
     size_t lut_bytes = 0, nz_bytes = 0, pixels = 0, flag_bytes = 0;
@@ -1465,7 +1424,7 @@ int hvr_batch_run_async(hvr_batch *b, int count)
     }
     // the graph for this shape
     std::string key = std::to_string(b->stage_cap) + "/" + std::to_string(lut_cap) + "/" +
-                      std::to_string(count) + "/" + std::to_string(b->tile_shape);
+                      std::to_string(count) + "/" + std::to_string(b->tile_shape) + "/" + std::to_string(variant);
     for (const Cls &k : cls)
         key += "/" + std::to_string(k.fmt) + "." + std::to_string(k.nm) + "." + std::to_string(k.lb) + "." +
                std::to_string(k.buf);
@@ -1482,7 +1441,7 @@ int hvr_batch_run_async(hvr_batch *b, int count)
             if (hipGetLastError() != hipSuccess) rc = HVR_EDEVICE;
         }
         for (size_t k = 0; k < cls.size() && rc == HVR_OK; ++k)
-            rc = batch_fn(cls[k].fmt, cls[k].nm, cls[k].lb, count != 0, b->tile_shape, cls[k].buf)(
+            rc = batch_fn(cls[k].fmt, cls[k].nm, cls[k].lb, count != 0, b->tile_shape, cls[k].buf, variant)(
                 b->d_stage, (int)k, b->d_counters + k, lut_cap, b->ncu, b->stream);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(b->stream, &g);
@@ -1655,7 +1614,7 @@ int hvr_render_async_to_noalloc(hvr_ctx *c, void *dst)
     hdr.all_offset = off.desc;
     hdr.cls[0].desc_offset = off.desc;
     hdr.cls[0].nreq = 1;
-    // the block tile of k_batch with 8x8 ray tiles: 16 x (4 * kBatchWaves / 2)
+    // the block tile of k_batch with 8x8 ray tiles and the default variant
     const int bx = 16, by = 8 * (kBatchWaves / 2);
     ReqDesc d;
     memset(&d, 0, sizeof(d));

@@ -615,7 +615,7 @@ __device__ __forceinline__ RayRec ray_setup(const FrameConst &fc, int px, int py
     return r;
 }
 
-__global__ void __launch_bounds__(256) k_ray_setup(const FrameConst fc, RayRec *__restrict__ rays)
+static __global__ void __launch_bounds__(256) k_ray_setup(const FrameConst fc, RayRec *__restrict__ rays)
 {
     const int idx = blockIdx.x * 256 + threadIdx.x;
     if (idx >= fc.W * fc.H) return;
@@ -994,7 +994,7 @@ __device__ __forceinline__ bool brick_clear(float2 r, const int *__restrict__ nz
     return brick_clear_range(r.x, r.y, nz, n);
 }
 
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 k_brick_flags(const float2 *__restrict__ range, uint32_t nbricks, const int *__restrict__ nz, int n,
               uint8_t *__restrict__ flags, uint32_t *__restrict__ n_clear)
 {
@@ -1179,7 +1179,7 @@ __device__ __forceinline__ double np_linspace(int i, int n)
     return (double)i * (2.0 / (double)(n - 1)) + -1.0;
 }
 
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 k_blob_field(float *__restrict__ out, int s0, int s1, int s2, const BlobSpec *__restrict__ spec, uint64_t nvox)
 {
     const uint64_t l = (uint64_t)blockIdx.x * 256 + threadIdx.x;
@@ -1247,12 +1247,11 @@ struct BatchHeader {
     int pad;
 };
 
-// Ray-tile shapes: a wave covers WX x WY pixels; a block is kBatchWaves
-// waves, 2 across and kBatchWaves / 2 down, and takes one tile of
-// (2 WX) x (kBatchWaves / 2 * WY) pixels at a time.  Measured on 24 frames
-// of the 512^3 volume and on the 64-request mix: 2 waves per block (half
-// the tile, twice the counter traffic) and 8 waves (a longer wait at the
-// per-tile barrier) are both slower than 4.
+// Ray-tile shapes: a wave covers WX x WY pixels; a block is W waves (2, 4
+// or 8; 4 by default), 2 across and W / 2 down, and takes one tile of
+// (2 WX) x (W / 2 * WY) pixels at a time.  Fewer waves per block halve the
+// tile and double the counter traffic; more lengthen the wait at the
+// per-tile barrier.  The autotuner times them.
 static constexpr int kBatchWaves = 4;
 
 template <int TS>
@@ -1266,13 +1265,17 @@ struct TileShape<1> {
     static constexpr int WX = 16, WY = 4;
 };
 
-template <int SF, int NM, int LB, bool COUNT, int TS, bool BUF>
-__global__ void __launch_bounds__(64 * kBatchWaves)
+// W: waves per block.  SKIP: step over transparent bricks (the flags of
+// k_batch_flags).  GATE: interpolate and light normals only for samples of
+// non-zero alpha.  Every variant gives the same image.
+template <int SF, int NM, int LB, bool COUNT, int TS, bool BUF, int W = kBatchWaves, bool SKIP = true,
+          bool GATE = true>
+__global__ void __launch_bounds__(64 * W)
 k_batch(const char *__restrict__ block, int cls, uint32_t *__restrict__ counter)
 {
     extern __shared__ float4 s_lut[];
     __shared__ int s_item[2];
-    constexpr int WX = TileShape<TS>::WX, WY = TileShape<TS>::WY, BX = 2 * WX, BY = kBatchWaves / 2 * WY;
+    constexpr int WX = TileShape<TS>::WX, WY = TileShape<TS>::WY, BX = 2 * WX, BY = W / 2 * WY;
     const BatchHeader *hdr = (const BatchHeader *)block;
     const int nreq = hdr->cls[cls].nreq;
     if (nreq <= 0) return;
@@ -1314,8 +1317,8 @@ k_batch(const char *__restrict__ block, int cls, uint32_t *__restrict__ counter)
             const RayRec r = ray_setup(R.fc, px, py);
             float C0 = 0.f, C1 = 0.f, C2 = 0.f, A = 0.f;
             uint32_t ns = 0;
-            march_ray<SF, NM, LB, COUNT, true, BUF, true>(R.fc, R.ts, R.ts, R.lut, R.lut_n, s_lut, r,
-                                                          R.flags, C0, C1, C2, A, ns);
+            march_ray<SF, NM, LB, COUNT, SKIP, BUF, GATE>(R.fc, R.ts, R.ts, R.lut, R.lut_n, s_lut, r,
+                                                           R.flags, C0, C1, C2, A, ns);
             const int pix = py * R.fc.W + px;
             st_global(R.out + pix, blend_store(R.fc, C0, C1, C2, A));
             if constexpr (COUNT) st_global(R.counts + pix, ns);
@@ -1326,7 +1329,7 @@ k_batch(const char *__restrict__ block, int cls, uint32_t *__restrict__ counter)
 }
 
 // Transparent-brick flags of every request of a batch for its own LUT.
-__global__ void __launch_bounds__(256) k_batch_flags(const char *__restrict__ block)
+static __global__ void __launch_bounds__(256) k_batch_flags(const char *__restrict__ block)
 {
     const BatchHeader *hdr = (const BatchHeader *)block;
     const ReqDesc *__restrict__ reqs = (const ReqDesc *)(block + hdr->all_offset);
@@ -1338,7 +1341,7 @@ __global__ void __launch_bounds__(256) k_batch_flags(const char *__restrict__ bl
     }
 }
 
-__global__ void __launch_bounds__(64) k_fill_u32(uint32_t *__restrict__ p, int n, uint32_t v)
+static __global__ void __launch_bounds__(64) k_fill_u32(uint32_t *__restrict__ p, int n, uint32_t v)
 {
     for (int i = threadIdx.x; i < n; i += 64) p[i] = v;
 }
@@ -1352,7 +1355,7 @@ struct StageOffsets {
     uint32_t counter, unbound_nz, desc;
 };
 
-__global__ void __launch_bounds__(64)
+static __global__ void __launch_bounds__(64)
 k_stage_request(char *__restrict__ blk, const StageOffsets off, const BatchHeader hdr, const ReqDesc desc)
 {
     const int lane = threadIdx.x;

@@ -43,6 +43,7 @@ EXPORTS = (
     "hvr_batch_counts", "hvr_batch_last_ms", "hvr_volume_upload_scalar_normals", "hvr_volume_generate_blobs",
     "hvr_batch_query", "hvr_device_memory", "hvr_read_pinned", "hvr_volume_download",
     "hvr_resize", "hvr_debug_inject_fault", "hvr_render_async_to_noalloc",
+    "hvr_batch_set_kernel",
 )
 
 # batch ray-tile shapes (hvr_batch_set_tile_shape)
@@ -113,6 +114,7 @@ def _declare(L):
         "hvr_batch_destroy": ([vp], i),
         "hvr_batch_clear": ([vp], i),
         "hvr_batch_set_tile_shape": ([vp, i], i),
+        "hvr_batch_set_kernel": ([vp, i, i, i], i),
         "hvr_batch_add": ([vp, vp, i, vp, i, fp, i, vp, i, i, ctypes.POINTER(i)], i),
         "hvr_batch_run_async": ([vp, i], i),
         "hvr_batch_wait": ([vp], i),

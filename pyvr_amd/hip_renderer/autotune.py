@@ -1,6 +1,9 @@
 """Autotuner for batched rendering: picks the brick size (8^3 or 16^3 interior
-texels) and the ray tile of a wavefront (8x8 or 16x4 pixels) by timing the
-persistent batch launch on the workload itself.
+texels), the ray tile of a wavefront (8x8 or 16x4 pixels) and the kernel
+variant (waves per block 2 / 4 / 8, transparent-brick skipping on or off,
+normals gated on non-zero alpha or not; ``batch.KERNEL_VARIANTS``) by timing
+the persistent batch launch on the workload itself.  Every candidate gives
+the same images; only the time differs.
 
 Each candidate renders the requests (all, or an evenly spaced sample of
 ``max_requests``) in one batch; the figure of merit is the median device time
@@ -15,7 +18,7 @@ from typing import Dict, Iterable, List, Optional, Sequence, Tuple
 
 import numpy as np
 
-from .batch import TILE_SHAPES, BatchRenderer
+from .batch import KERNEL_VARIANTS, TILE_SHAPES, BatchRenderer
 
 BRICK_LOG2 = (3, 4)
 
@@ -43,10 +46,12 @@ def _sample(requests: Sequence, max_requests: int) -> List:
 
 def tune(volumes: Sequence, requests: Sequence, scalar_format: str = "f16", normal_format: str = "sn8x3",
          runs: int = 3, max_requests: int = 64, device: int = 0, cache: Optional[str] = None,
-         bricks: Iterable[int] = BRICK_LOG2, tiles: Iterable[str] = tuple(TILE_SHAPES)) -> Dict:
-    """Time every (brick_log2, tile_shape) candidate; return
-    ``{"brick_log2", "tile_shape", "table": [(brick_log2, tile_shape, ms), ...],
-    "signature", "cached"}``.  ``requests`` are objects with ``volume``
+         bricks: Iterable[int] = BRICK_LOG2, tiles: Iterable[str] = tuple(TILE_SHAPES),
+         kernels: Iterable[str] = tuple(KERNEL_VARIANTS)) -> Dict:
+    """Time every (brick_log2, tile_shape, kernel) candidate; return
+    ``{"brick_log2", "tile_shape", "kernel", "ms", "table": [(brick_log2,
+    tile_shape, kernel, ms), ...], "signature", "cached"}``: the candidate
+    with the smallest median time.  ``requests`` are objects with ``volume``
     (index into ``volumes``), ``camera``, ``ctf``, ``otf``, ``width``,
     ``height``, ``config``, ``light`` and ``tau``."""
     key = signature(volumes, requests, scalar_format, normal_format)
@@ -59,10 +64,11 @@ def tune(volumes: Sequence, requests: Sequence, scalar_format: str = "f16", norm
             table = {}
         if key in table:
             out = dict(table[key])
+            out["table"] = [tuple(t) for t in out["table"]]
             out["cached"] = True
             return out
     sample = _sample(requests, max_requests)
-    timings: List[Tuple[int, str, float]] = []
+    timings: List[Tuple[int, str, str, float]] = []
     for lb in bricks:
         b = BatchRenderer(device=device, scalar_format=scalar_format, normal_format=normal_format,
                           brick_log2=lb, pipeline=1)
@@ -71,16 +77,19 @@ def tune(volumes: Sequence, requests: Sequence, scalar_format: str = "f16", norm
                 b.add(volumes[q.volume], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
             for tile in tiles:
                 b.set_tile_shape(tile)
-                b.run()
-                ms = []
-                for _ in range(runs):
+                for kernel in kernels:
+                    b.set_kernel(kernel)
                     b.run()
-                    ms.append(b.last_ms())
-                timings.append((lb, tile, float(np.median(ms))))
+                    ms = []
+                    for _ in range(runs):
+                        b.run()
+                        ms.append(b.last_ms())
+                    timings.append((lb, tile, kernel, float(np.median(ms))))
         finally:
             b.close()
-    lb, tile, _ = min(timings, key=lambda t: t[2])
-    out = {"brick_log2": lb, "tile_shape": tile, "table": timings, "signature": key, "cached": False}
+    lb, tile, kernel, best = min(timings, key=lambda t: t[3])
+    out = {"brick_log2": lb, "tile_shape": tile, "kernel": kernel, "ms": best, "table": timings,
+           "signature": key, "cached": False}
     if cache:
         table[key] = {k: v for k, v in out.items() if k != "cached"}
         os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)

@@ -145,14 +145,22 @@ class ResidentVolume:
         self.device.destroy()
 
 
+# Kernel variants of the batched march (hvr_batch_set_kernel): waves per
+# block, transparent-brick skipping, normals gated on non-zero alpha.
+KERNEL_VARIANTS = {"default": (4, True, True), "waves2": (2, True, True), "waves8": (8, True, True),
+                   "nogate": (4, True, False), "noskip": (4, False, True)}
+
+
 class _Slot:
     """One ``hvr_batch`` (own stream and buffers) and its requests' sizes."""
 
-    def __init__(self, L, ctx, tile_shape):
+    def __init__(self, L, ctx, tile_shape, kernel="default"):
         self.L = L
         self.handle = ctypes.c_void_p()
         check(L.hvr_batch_create(ctx, ctypes.byref(self.handle)), "hvr_batch_create")
         check(L.hvr_batch_set_tile_shape(self.handle, TILE_SHAPES[tile_shape]), "hvr_batch_set_tile_shape")
+        w, sk, g = KERNEL_VARIANTS[kernel]
+        check(L.hvr_batch_set_kernel(self.handle, w, int(sk), int(g)), "hvr_batch_set_kernel")
         self.sizes: List[tuple] = []
         self.in_flight = False
 
@@ -168,8 +176,10 @@ class BatchRenderer:
     ``scalar_format`` / ``normal_format`` select the device storage of the
     volumes it uploads ("f32"/"f16"/"u8", "f32x3"/"sn8x3", or "f16x3"
     with "f16"); ``brick_log2``
-    the brick edge (3: 8^3, 4: 16^3) and ``tile_shape`` the ray tile of a
-    wavefront ("8x8" or "16x4") -- the choices the autotuner makes.
+    the brick edge (3: 8^3, 4: 16^3), ``tile_shape`` the ray tile of a
+    wavefront ("8x8" or "16x4") and ``kernel`` the kernel variant (a key of
+    ``KERNEL_VARIANTS``: waves per block, skipping, gating; every variant
+    gives the same images) -- the choices the autotuner makes.
 
     ``pipeline`` batches can be in flight at once (each has its own stream
     and buffers): ``submit()`` launches the batch being filled and moves on
@@ -178,9 +188,11 @@ class BatchRenderer:
     """
 
     def __init__(self, device: int = 0, scalar_format: str = "f16", normal_format: str = "sn8x3",
-                 brick_log2: int = 3, tile_shape: str = "8x8", pipeline: int = 2):
+                 brick_log2: int = 3, tile_shape: str = "8x8", pipeline: int = 2, kernel: str = "default"):
         if tile_shape not in TILE_SHAPES:
             raise ValueError(f"tile_shape must be one of {sorted(TILE_SHAPES)}")
+        if kernel not in KERNEL_VARIANTS:
+            raise ValueError(f"kernel must be one of {sorted(KERNEL_VARIANTS)}")
         if pipeline < 1:
             raise ValueError("pipeline must be at least 1")
         self.L = _lib.lib()
@@ -188,6 +200,7 @@ class BatchRenderer:
         self.normal_format = normal_format
         self.brick_log2 = brick_log2
         self.tile_shape = tile_shape
+        self.kernel = kernel
         self.ctx = ctypes.c_void_p()
         check(self.L.hvr_create(device, 1, 1, ctypes.byref(self.ctx)), "hvr_create")
         self._resident: Dict[int, ResidentVolume] = {}
@@ -195,13 +208,22 @@ class BatchRenderer:
         try:
             check(self.L.hvr_set_brick_log2(self.ctx, brick_log2), "hvr_set_brick_log2")
             for _ in range(pipeline):
-                self._slots.append(_Slot(self.L, self.ctx, tile_shape))
+                self._slots.append(_Slot(self.L, self.ctx, tile_shape, kernel))
         except Exception:
             self.close()
             raise
         self._cur = 0        # slot being filled
         self._last = None    # slot of the last run
         self._reuse = False  # the slot being filled still holds submitted requests
+
+    def set_kernel(self, kernel: str):
+        """Kernel variant for the following launches (a key of KERNEL_VARIANTS)."""
+        if kernel not in KERNEL_VARIANTS:
+            raise ValueError(f"kernel must be one of {sorted(KERNEL_VARIANTS)}")
+        w, sk, g = KERNEL_VARIANTS[kernel]
+        for s in self._slots:
+            check(self.L.hvr_batch_set_kernel(s.handle, w, int(sk), int(g)), "hvr_batch_set_kernel")
+        self.kernel = kernel
 
     def set_tile_shape(self, tile_shape: str):
         """Ray tile of a wavefront for the following launches ("8x8" or "16x4")."""

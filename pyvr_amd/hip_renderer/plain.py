@@ -24,6 +24,7 @@ from ..lighting import Light
 from . import _lib
 from ._lib import check
 from .batch import DeviceBlobVolume, ResidentVolume, request_params, rgba_lut  # noqa: F401 (re-export)
+from .manager import _bytes_data, _new_bytes
 
 
 class PlainRenderer:
@@ -37,6 +38,7 @@ class PlainRenderer:
         self._resident: Dict[int, ResidentVolume] = {}
         self._bound = None      # handle of the bound volume
         self._lut = None        # bytes of the LUT last set
+        self._last = None       # (volume, config, light, threshold) of the prepared request
         try:
             check(self.L.hvr_set_brick_log2(self.ctx, brick_log2), "hvr_set_brick_log2")
         except Exception:
@@ -85,15 +87,46 @@ class PlainRenderer:
             self._lut = lut.tobytes()
         self.params = request_params(volume, camera, config, light, width, height, opacity_threshold)
         check(self.L.hvr_set_params(self.ctx, ctypes.byref(self.params)), "hvr_set_params")
+        self._last = (volume, config, light, opacity_threshold)
+
+    def set_camera(self, camera):
+        """Only the camera of the prepared request changes (what
+        ``VolumeRenderer.set_camera`` does between frames)."""
+        if self._last is None:
+            raise RuntimeError("no request prepared")
+        volume, config, light, threshold = self._last
+        w, h = self.size
+        self.params = request_params(volume, camera, config, light, w, h, threshold)
+        check(self.L.hvr_set_params(self.ctx, ctypes.byref(self.params)), "hvr_set_params")
+
+    def read(self) -> bytes:
+        """One frame of the prepared request, read back into a new bytes
+        object (width*height*4 RGBA8, bottom row first), as
+        ``VolumeRenderer.render()`` returns it."""
+        w, h = self.size
+        check(self.L.hvr_render_async(self.ctx), "hvr_render_async")
+        out = _new_bytes(None, w * h * 4)
+        check(self.L.hvr_read(self.ctx, ctypes.cast(_bytes_data(out), ctypes.POINTER(ctypes.c_uint8))), "hvr_read")
+        return out
+
+    def read_pinned(self) -> np.ndarray:
+        """One frame of the prepared request, copied to the context's pinned
+        host buffer: a read-only uint8 view (width*height*4) valid until the
+        next frame -- the pixels in host memory without a further copy, as a
+        served request's are."""
+        w, h = self.size
+        check(self.L.hvr_render_async(self.ctx), "hvr_render_async")
+        ptr = ctypes.POINTER(ctypes.c_uint8)()
+        check(self.L.hvr_read_pinned(self.ctx, ctypes.byref(ptr)), "hvr_read_pinned")
+        a = np.ctypeslib.as_array(ptr, shape=(w * h * 4,))
+        a.flags.writeable = False
+        return a
 
     def render(self, *args, **kw) -> bytes:
         """``prepare`` then one frame, read back: width*height*4 RGBA8 bytes,
         bottom row first (what ``VolumeRenderer.render()`` returns)."""
         self.prepare(*args, **kw)
-        w, h = self.size
-        out = ctypes.create_string_buffer(w * h * 4)
-        check(self.L.hvr_render(self.ctx, ctypes.cast(out, ctypes.POINTER(ctypes.c_uint8))), "hvr_render")
-        return out.raw
+        return self.read()
 
     def launch_to(self, device_rgba8):
         """Enqueue the prepared request's frame into a device buffer of

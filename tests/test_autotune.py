@@ -8,13 +8,14 @@ import numpy as np
 from pyvr_amd.hip_renderer import autotune
 
 TIMES = {(3, "8x8"): 5.0, (3, "16x4"): 4.0, (4, "8x8"): 6.0, (4, "16x4"): 3.5}
+KERNEL_FACTOR = {"default": 1.0, "waves2": 1.1, "waves8": 0.97, "nogate": 1.05, "noskip": 1.2}
 
 
 class FakeBatch:
     made = []
 
     def __init__(self, device=0, scalar_format="f16", normal_format="sn8x3", brick_log2=3, pipeline=1):
-        self.lb, self.tile, self.n = brick_log2, "8x8", 0
+        self.lb, self.tile, self.kernel, self.n = brick_log2, "8x8", "default", 0
         FakeBatch.made.append(self)
 
     def add(self, *a, **k):
@@ -23,11 +24,14 @@ class FakeBatch:
     def set_tile_shape(self, t):
         self.tile = t
 
+    def set_kernel(self, k):
+        self.kernel = k
+
     def run(self):
         pass
 
     def last_ms(self):
-        return TIMES[self.lb, self.tile]
+        return TIMES[self.lb, self.tile] * KERNEL_FACTOR[self.kernel]
 
     def close(self):
         pass
@@ -47,12 +51,14 @@ def test_picks_fastest_and_caches(monkeypatch, tmp_path):
     vols, reqs = workload(10)
     cache = str(tmp_path / "tune.json")
     out = autotune.tune(vols, reqs, cache=cache)
-    assert (out["brick_log2"], out["tile_shape"]) == (4, "16x4")
-    assert not out["cached"] and len(out["table"]) == 4
+    assert (out["brick_log2"], out["tile_shape"], out["kernel"]) == (4, "16x4", "waves8")
+    assert out["ms"] == min(t[3] for t in out["table"])
+    assert not out["cached"] and len(out["table"]) == 4 * 5
     assert {b.lb for b in FakeBatch.made} == {3, 4} and all(b.n == 10 for b in FakeBatch.made)
     FakeBatch.made = []
     again = autotune.tune(vols, reqs, cache=cache)
-    assert again["cached"] and (again["brick_log2"], again["tile_shape"]) == (4, "16x4")
+    assert again["cached"] and (again["brick_log2"], again["tile_shape"], again["kernel"]) == (4, "16x4", "waves8")
+    assert again["table"] == out["table"]
     assert FakeBatch.made == []                       # nothing timed on a cache hit
     # another workload misses the cache
     vols2, reqs2 = workload(11)

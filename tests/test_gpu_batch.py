@@ -206,3 +206,21 @@ def test_batch_rejects_unbatchable_requests(hip):
         assert len(b.result(0)) == 32 * 32 * 4
     finally:
         b.close()
+
+
+@pytest.mark.parametrize("kernel", ["waves2", "waves8", "nogate", "noskip"])
+def test_batch_kernel_variants_equal_default(hip, kernel):
+    """Every kernel variant the autotuner may pick (waves per block,
+    skipping, gating) renders the same bytes as the default variant, which
+    the tests above hold equal to single renders."""
+    vols = scenes.c3_volumes(scale=4)
+    reqs = scenes.c3_requests(10, seed=17, image_scale=4)
+    out = {}
+    for k in ("default", kernel):
+        b, idx = make_batch(vols, reqs, kernel=k)
+        try:
+            b.run()
+            out[k] = [b.result(i) for i in idx]
+        finally:
+            b.close()
+    assert out[kernel] == out["default"]

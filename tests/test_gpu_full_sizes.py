@@ -84,9 +84,10 @@ def test_c3_batch_full_size_against_oracle(hip):
     reqs = scenes.c3_requests(64)
     print(f"[c3] volumes {[v.data.shape for v in vols]} in {time.perf_counter() - t0:.1f}s", flush=True)
     tuning = tune(vols, reqs, "f16", "sn8x3")
-    lb, tile = tuning["brick_log2"], tuning["tile_shape"]
-    print(f"[c3] autotuner: {tuning['table']} -> lb{lb}/{tile}", flush=True)
-    b = BatchRenderer(scalar_format="f16", normal_format="sn8x3", brick_log2=lb, tile_shape=tile, pipeline=1)
+    lb, tile, kernel = tuning["brick_log2"], tuning["tile_shape"], tuning["kernel"]
+    print(f"[c3] autotuner: {tuning['table']} -> lb{lb}/{tile}/{kernel}", flush=True)
+    b = BatchRenderer(scalar_format="f16", normal_format="sn8x3", brick_log2=lb, tile_shape=tile, pipeline=1,
+                      kernel=kernel)
     try:
         idx = [b.add(vols[q.volume], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
                for q in reqs]
