@@ -131,7 +131,8 @@ def kernel_source_hash() -> str:
     import hashlib
 
     h = hashlib.sha256()
-    for rel in ("pyvr_amd/csrc/hvr_device.h", "pyvr_amd/csrc/hvr_rules.h", "pyvr_amd/csrc/hipvr.hip"):
+    for rel in ("pyvr_amd/csrc/hvr_device.h", "pyvr_amd/csrc/hvr_rules.h", "pyvr_amd/csrc/hvr_launch.h",
+                "pyvr_amd/csrc/inst.hip", "pyvr_amd/csrc/hipvr.hip"):
         with open(os.path.join(ROOT, rel), "rb") as f:
             h.update(f.read())
     return h.hexdigest()[:16]
@@ -221,6 +222,26 @@ def cpu_baseline(wl: Workload, data, normals, seconds):
 # ---------------------------------------------------------------------------
 # legs
 
+def volume_renderer(wl: Workload, stored, sf, nf):
+    """A VolumeRenderer of the workload's size holding the stored values of
+    the workload's volume (read back from the device: FP16 values are
+    re-stored exactly; SNORM8 normals come back as integers and are scaled to
+    [-1, 1] so that they re-quantise to the same integers), with the
+    workload's transfer functions and termination threshold."""
+    from pyvr_amd.hip_renderer import VolumeRenderer
+    from pyvr_amd.volume import Volume
+
+    scalar, normals = stored
+    if normals is not None and nf == "sn8x3":
+        normals = normals / np.float32(127.0)
+    vr = VolumeRenderer(wl.width, wl.height, config=wl.config, light=wl.light, scalar_format=sf, normal_format=nf)
+    vr.load_volume(Volume(data=scalar, normals=normals, min_bounds=np.asarray(wl.volume.min_bounds),
+                          max_bounds=np.asarray(wl.volume.max_bounds)))
+    vr.set_transfer_functions(wl.ctf, wl.otf)
+    vr.set_opacity_threshold(wl.tau)
+    return vr
+
+
 def plain_leg(wl: Workload, args, sf, nf):
     """One launch per frame (ray setup + march) into resident device
     buffers, back to back on one stream; then end-to-end per-frame calls."""
@@ -288,20 +309,26 @@ def plain_leg(wl: Workload, args, sf, nf):
     for b in bufs:
         L.hvr_device_free(ctx, b)
 
-    # end-to-end frames as VolumeRenderer.set_camera + render() make them:
-    # camera update, draw, RGBA8 copy into a new bytes object
-    lat = []
-    pr.prepare(wl.volume, wl.cams[0], wl.ctf, wl.otf, wl.width, wl.height, wl.config, wl.light, wl.tau)
-    for i in range(args.latency_frames + 2):
-        cam = wl.cams[i % len(wl.cams)]
-        t = time.perf_counter()
-        pr.set_camera(cam)
-        pr.read()
-        if i >= 2:
-            lat.append((time.perf_counter() - t) * 1e3)
-    lat = np.array(lat)
-    stored = pr.download(wl.volume) if not args.no_cpu_baseline else (None, None)
+    # the volume's stored values, read back (for the per-frame API and the
+    # CPU baseline)
+    stored = pr.download(wl.volume)
     pr.close()
+    # end-to-end frames through the reference API: VolumeRenderer.set_camera
+    # + render() (camera update, draw, RGBA8 copy into a new bytes object),
+    # on the same stored texels
+    lat = []
+    vr = volume_renderer(wl, stored, sf, nf)
+    try:
+        for i in range(args.latency_frames + 2):
+            cam = wl.cams[i % len(wl.cams)]
+            t = time.perf_counter()
+            vr.set_camera(cam)
+            vr.render()
+            if i >= 2:
+                lat.append((time.perf_counter() - t) * 1e3)
+    finally:
+        vr.close()
+    lat = np.array(lat)
     bpv = 8 * (SCALAR_BYTES[sf] + (NORMAL_BYTES[nf] if wl.has_normals else 0))
     key = f"{wl.name}/{len(wl.cams)}cams/{sf}+{nf if wl.has_normals else 'none'}/plain/lb{lb}/skip{int(used.value)}"
     roofline = roofline_of("hvr::k_march", s_step / len(wl.cams), bpv, 4 * npix, march_ms.value / nfr.value, key)

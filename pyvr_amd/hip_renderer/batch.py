@@ -56,9 +56,10 @@ def request_params(volume: Volume, camera: Camera, config: RenderConfig, light: 
     """``hvr_params`` of one request, converted exactly as the renderer's
     uniform setters convert them (float32 matrices and vectors)."""
     p = HvrParams()
-    view = np.asarray(camera.get_view_matrix(), dtype=np.float32).reshape(4, 4)
+    # the camera vectors once (get_view_matrix would compute them again)
+    position, up = camera.get_camera_vectors()
+    view = np.asarray(camera._view_matrix(position, up), dtype=np.float32).reshape(4, 4)
     proj = np.asarray(camera.get_projection_matrix(width / height), dtype=np.float32).reshape(4, 4)
-    position, _ = camera.get_camera_vectors()
     p.view[:] = view.ravel().tolist()
     p.proj[:] = proj.ravel().tolist()
     p.cam_pos[:] = _vec3(position)

@@ -53,6 +53,10 @@ def test_picks_fastest_and_caches(monkeypatch, tmp_path):
     out = autotune.tune(vols, reqs, cache=cache)
     assert (out["brick_log2"], out["tile_shape"], out["kernel"]) == (4, "16x4", "waves8")
     assert out["ms"] == min(t[3] for t in out["table"])
+    # a best candidate within the margin of the first (default) one is not taken
+    near = autotune.tune(vols, reqs, bricks=(3,), tiles=("8x8",))
+    assert (near["brick_log2"], near["tile_shape"], near["kernel"]) == (3, "8x8", "default")
+    assert near["ms"] == 5.0
     assert not out["cached"] and len(out["table"]) == 4 * 5
     assert {b.lb for b in FakeBatch.made} == {3, 4} and all(b.n == 10 for b in FakeBatch.made)
     FakeBatch.made = []

@@ -1,9 +1,10 @@
 """The autotuner on the GPU, on the batched 64-request mix (FP16 +
 SNORM8x3 volumes of 64^3 .. 256^3 and 96x160x224 at half size, images of
 128^2 .. 512^2): it times every candidate (brick size x ray tile x kernel
-variant), its pick is the fastest of them and stays at least as fast as the
-default configuration when timed again, and the picked configuration's
-images equal the default configuration's byte for byte."""
+variant), its pick is the fastest of them (or the default when nothing beats
+it by more than 5 %) and stays at least as fast as the default configuration
+when timed again, and the picked configuration's images equal the default
+configuration's byte for byte."""
 
 import numpy as np
 import pytest
@@ -43,7 +44,8 @@ def test_autotuner_pick_is_fastest_and_renders_the_default_images(hip, tmp_path)
           ", ".join(f"lb{a}/{t}/{k} {ms:.3f}" for a, t, k, ms in table) +
           f" -> lb{out['brick_log2']}/{out['tile_shape']}/{out['kernel']}", flush=True)
     assert len(table) == 2 * 2 * 5
-    assert out["ms"] == min(t[3] for t in table)
+    best = min(t[3] for t in table)
+    assert out["ms"] == best or (out["ms"] == table[0][3] and best >= 0.95 * table[0][3])
     pick = (out["brick_log2"], out["tile_shape"], out["kernel"])
     got, ms_pick = _render_and_time(vols, reqs, *pick)
     want, ms_default = _render_and_time(vols, reqs, 3, "8x8", "default")
