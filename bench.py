@@ -64,6 +64,8 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
 VALU_PEAK = 1024 * 2.4e9 / 4 * 64   # lane instructions/s: 1024 SIMDs, wave64 per 4 cycles, 2.4 GHz
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+ORDER_MODES = {"off": 0, "on": 1, "auto": 2}   # hvr_set_dispatch_order (HVR_ORDER_OFF / _ON / _AUTO)
+READBACK_MODES = {"copy": 0, "direct": 1}      # hvr_set_readback (HVR_READBACK_COPY / _DIRECT)
 SCALAR_BYTES = {"f32": 4, "f16": 2, "u8": 1}
 NORMAL_BYTES = {"f32x3": 12, "sn8x3": 3, "f16x3": 6, "none": 0}
 METRIC = "ray samples/s + fps at 512^3/1024^2; p50/p99 frame latency vs plain kernel"
@@ -152,7 +154,7 @@ def measured_profile(key: str) -> dict:
     return e
 
 
-def roofline_of(kernel, samples, bpv, out_bytes, ms, key):
+def roofline_of(kernel, samples, bpv, out_bytes, ms, key, shaded_fraction=None):
     """Roofline object of one launch of ``kernel``: measured HBM bytes per
     launch over its duration against HBM peak (``frac``), the algorithmic
     bytes beside it (``alg``), and the VALU issue roof (``valu``) from the
@@ -172,6 +174,12 @@ def roofline_of(kernel, samples, bpv, out_bytes, ms, key):
                 "frac": alg_bytes / t / HBM_PEAK},
         "limiter": None,
     }
+    if shaded_fraction is not None:
+        # the same count over the samples that load texels (those in
+        # transparent bricks are stepped over), from the counting pass
+        shaded = samples * shaded_fraction * bpv + out_bytes
+        out["alg"].update({"shaded_fraction": shaded_fraction, "shaded_bytes_per_launch": shaded,
+                           "shaded_achieved": shaded / t / 1e9, "shaded_frac": shaded / t / HBM_PEAK})
     if "valu_per_sample" in prof:
         va = prof["valu_per_sample"] * samples / t
         out["valu"] = {"instr_per_sample": prof["valu_per_sample"], "achieved": va / 1e12, "peak": VALU_PEAK / 1e12,
@@ -254,16 +262,19 @@ def plain_leg(wl: Workload, args, sf, nf):
     ctx = pr.ctx
     _lib.check(L.hvr_set_empty_skip(ctx, {"off": _lib.SKIP_OFF, "on": _lib.SKIP_ON, "auto": _lib.SKIP_AUTO}[args.skip]),
                "set_empty_skip")
+    _lib.check(L.hvr_set_dispatch_order(ctx, ORDER_MODES[args.order]), "set_dispatch_order")
     t0 = time.perf_counter()
     pr.upload(wl.volume)
     vol_bytes = pr.resident_bytes()
     log(f"volume resident: {vol_bytes / 2**20:.0f} MiB in {time.perf_counter() - t0:.1f}s")
-    params, counts = [], []
+    params, counts, shaded = [], [], []
     for cam in wl.cams:
         pr.prepare(wl.volume, cam, wl.ctf, wl.otf, wl.width, wl.height, wl.config, wl.light, wl.tau)
         params.append(pr.params)
         counts.append(int(pr.sample_counts().sum()))
+        shaded.append(int(pr.shaded_sample_counts().sum()))
     s_step = sum(counts)
+    shaded_fraction = sum(shaded) / max(1, s_step)
     npix = wl.width * wl.height
     log(f"samples per step: {s_step} ({len(wl.cams)} frames)")
     bufs = []
@@ -306,6 +317,9 @@ def plain_leg(wl: Workload, args, sf, nf):
     clear_b, total_b, used = ctypes.c_ulonglong(), ctypes.c_ulonglong(), ctypes.c_int()
     _lib.check(L.hvr_skip_info(ctx, ctypes.byref(clear_b), ctypes.byref(total_b), ctypes.byref(used)), "skip_info")
     skip = {"mode": args.skip, "used": bool(used.value), "clear_bricks": clear_b.value, "bricks": total_b.value}
+    ordered = ctypes.c_int()
+    _lib.check(L.hvr_dispatch_order_info(ctx, ctypes.byref(ordered)), "dispatch_order_info")
+    order = {"mode": args.order, "used": bool(ordered.value), "readback": args.readback}
     for b in bufs:
         L.hvr_device_free(ctx, b)
 
@@ -318,6 +332,8 @@ def plain_leg(wl: Workload, args, sf, nf):
     # on the same stored texels
     lat = []
     vr = volume_renderer(wl, stored, sf, nf)
+    _lib.check(L.hvr_set_dispatch_order(vr.gl_manager.ctx, ORDER_MODES[args.order]), "set_dispatch_order")
+    _lib.check(L.hvr_set_readback(vr.gl_manager.ctx, READBACK_MODES[args.readback]), "set_readback")
     try:
         for i in range(args.latency_frames + 2):
             cam = wl.cams[i % len(wl.cams)]
@@ -331,11 +347,12 @@ def plain_leg(wl: Workload, args, sf, nf):
     lat = np.array(lat)
     bpv = 8 * (SCALAR_BYTES[sf] + (NORMAL_BYTES[nf] if wl.has_normals else 0))
     key = f"{wl.name}/{len(wl.cams)}cams/{sf}+{nf if wl.has_normals else 'none'}/plain/lb{lb}/skip{int(used.value)}"
-    roofline = roofline_of("hvr::k_march", s_step / len(wl.cams), bpv, 4 * npix, march_ms.value / nfr.value, key)
+    roofline = roofline_of("hvr::k_march", s_step / len(wl.cams), bpv, 4 * npix, march_ms.value / nfr.value, key,
+                           shaded_fraction)
     return {"value": value, "ms_per_step": ms_step, "samples": s_step, "march_kernel_ms": march_ms.value / nfr.value,
             "fps": 1e3 / float(lat.mean()), "frame_ms_p50": float(np.percentile(lat, 50)),
             "frame_ms_p99": float(np.percentile(lat, 99)), "frames_per_s_device": len(wl.cams) * args.steps / wall,
-            "roofline": roofline, "brick_skip": skip, "volume_bytes": vol_bytes, "bpv": bpv}, stored
+            "roofline": roofline, "shaded_fraction": shaded_fraction, "brick_skip": skip, "dispatch_order": order, "volume_bytes": vol_bytes, "bpv": bpv}, stored
 
 
 def batched_leg(wl: Workload, args, sf, nf, s_step):
@@ -428,7 +445,8 @@ def serving_leg(args, rates, duration, nvol, plain_requests):
     vols = scenes.c4_volumes(nvol)
     reqs = scenes.c4_requests(n_max, nvol=nvol)
     srv = RenderServer(scalar_format=sf, normal_format=nf, brick_log2=lb, tile_shape=tile,
-                       max_batch_requests=args.serve_max_requests, max_batch_pixels=1 << 26, pipeline=3,
+                       max_batch_requests=args.serve_max_requests, max_batch_pixels=1 << 26,
+                       pipeline=args.serve_pipeline,
                        keep_results=False, max_batch_samples=args.serve_max_samples or None)
     t0 = time.perf_counter()
     for k, v in enumerate(vols):
@@ -514,7 +532,8 @@ def serving_leg(args, rates, duration, nvol, plain_requests):
         f"{plain['samples_per_s'] / 1e9:.1f} G samples/s")
     best = max(table, key=lambda r: r["samples_per_s"])
     return {"volumes": nvol, "scalar_format": sf, "normal_format": nf, "brick_log2": lb, "tile_shape": tile,
-            "max_batch_samples": args.serve_max_samples, "mean_samples_per_request": float(samples.mean()),
+            "max_batch_samples": args.serve_max_samples, "pipeline": args.serve_pipeline,
+            "mean_samples_per_request": float(samples.mean()),
             "rates": table, "plain": plain, "served_vs_plain": best["samples_per_s"] / plain["samples_per_s"]}
 
 
@@ -682,12 +701,18 @@ def main():
     ap.add_argument("--serve-rates", default="500,1000", help="serving leg of the default run (req/s)")
     ap.add_argument("--serve-duration", type=float, default=1.0, help="serving leg: seconds of arrivals per rate")
     ap.add_argument("--serve-max-requests", type=int, default=64)
+    ap.add_argument("--serve-pipeline", type=int, default=3, help="serving: batches in flight")
     ap.add_argument("--serve-max-samples", type=float, default=2e9,
                     help="serving: bound on a batch's estimated samples (0: requests and pixels only)")
     ap.add_argument("--no-serve", action="store_true", help="default run without the serving leg")
     ap.add_argument("--brick-log2", type=int, default=-1)
     ap.add_argument("--skip", choices=("auto", "on", "off"), default="auto",
                     help="transparent-brick skipping mode (result is identical in every mode)")
+    ap.add_argument("--order", choices=tuple(ORDER_MODES), default="auto",
+                    help="single-frame dispatch order: cost-ordered blocks (on), row order (off), or by frame size")
+    ap.add_argument("--readback", choices=tuple(READBACK_MODES), default="direct",
+                    help="per-frame latency leg: frames copied out of the device framebuffer, or written by the "
+                         "march straight into pinned host memory")
     ap.add_argument("--latency-frames", type=int, default=48)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -719,8 +744,12 @@ def main():
         batched = batched_leg(wl, args, sf, nf, s_step)
         bkey = (f"{wl.name}/{len(wl.cams)}cams/{sf}+{nf}/batch/lb{batched['brick_log2']}/{batched['tile_shape']}"
                 f"/{batched['kernel']}")
+        # the skipping variants step over the same samples as the single-frame
+        # kernel; the noskip variant loads texels for every sample
+        sh = plain["shaded_fraction"] if batched["kernel"] != "noskip" else 1.0
         roofline = roofline_of("hvr::k_batch (graph launch: staging copy, counter fill, brick flags, k_batch)",
-                               batched["samples"], bpv, 4 * npix * len(wl.cams), batched["device_ms_per_run"], bkey)
+                               batched["samples"], bpv, 4 * npix * len(wl.cams), batched["device_ms_per_run"], bkey,
+                               sh)
     serving = None
     if args.workload == "c2" and not args.plain_only and not args.no_serve:
         serving = serving_leg(args, [float(r) for r in args.serve_rates.split(",")], args.serve_duration,
@@ -730,7 +759,7 @@ def main():
         cpu = cpu_baseline(wl, stored[0], stored[1], args.cpu_seconds)
 
     plain_out = {k: plain[k] for k in ("value", "ms_per_step", "march_kernel_ms", "fps", "frame_ms_p50",
-                                       "frame_ms_p99", "frames_per_s_device", "roofline")}
+                                       "frame_ms_p99", "frames_per_s_device", "dispatch_order", "roofline")}
     if batched is not None:
         extra = {
             "fps": batched["frames_per_s"],

@@ -126,8 +126,9 @@ int hvr_destroy(hvr_ctx *ctx);
  * renderer keeps the size it was created with: a new size there means a new
  * ModernGLManager, manager.py:14-41).  Volumes, LUT and bindings stay; the
  * parameters' matrices are the caller's (aspect = width / height); buffers
- * grow when the new size needs more pixels and are kept otherwise.
- * Waits for the context's stream. */
+ * grow when the new size needs more pixels and are kept otherwise; only
+ * growing waits for the context's stream (frames in flight keep their own
+ * sizes). */
 int hvr_resize(hvr_ctx *ctx, int width, int height);
 
 /* ---- volumes: replace create_volume_texture (manager.py:77-108) and
@@ -177,7 +178,9 @@ int hvr_volume_destroy(hvr_volume *vol);
 /* ---- transfer function: replaces create_rgba_transfer_function_texture
  * (manager.py:137-186) ----  `rgba` is n x 4 float32 (RGB from the colour
  * LUT, A from the opacity LUT).  n == 0 unbinds the LUT (an unbound sampler
- * reads (0, 0, 0, 1)). */
+ * reads (0, 0, 0, 1)).  The copy is queued on the context's stream behind
+ * the frames already enqueued (through pinned staging), so a LUT change
+ * does not wait for them; a LUT larger than any before it does. */
 int hvr_set_lut(hvr_ctx *ctx, const float *rgba, int n);
 
 /* ---- uniforms: replace set_uniform_* (manager.py:188-210) ---- */
@@ -202,26 +205,45 @@ int hvr_bind(hvr_ctx *ctx, hvr_volume *scalar_vol, int scalar_role, hvr_volume *
 int hvr_render(hvr_ctx *ctx, uint8_t *out);
 
 /* Asynchronous form: enqueue the frame on the context's stream, leaving the
- * image in the context's device framebuffer.  hvr_read copies it out (and
- * waits). */
+ * image in the context's device framebuffer (or, with HVR_READBACK_DIRECT,
+ * its pinned host buffer).  hvr_read copies it out (and waits). */
 int hvr_render_async(hvr_ctx *ctx);
 int hvr_read(hvr_ctx *ctx, uint8_t *out);
 int hvr_synchronize(hvr_ctx *ctx);
 
 /* The read step without a host copy (replaces fbo.read of read_pixels,
  * manager.py:228-230): copies the framebuffer into the context's pinned host
- * buffer, waits, and sets *rgba to it (width*height*4 bytes, bottom row
+ * buffer (HVR_READBACK_DIRECT: the frame is already there), waits, and sets
+ * *rgba to it (width*height*4 bytes, bottom row
  * first; valid until the next read or render on this context), so that a
  * caller building its own object (a Python bytes) copies the image once. */
 int hvr_read_pinned(hvr_ctx *ctx, const uint8_t **rgba);
 
-/* Device pointer of the context's RGBA8 framebuffer (width*height*4 bytes). */
+/* Device pointer of the context's RGBA8 framebuffer (width*height*4 bytes;
+ * frames of HVR_READBACK_DIRECT are not written there). */
 int hvr_framebuffer(hvr_ctx *ctx, void **device_ptr);
+
+/* Where frames of hvr_render / hvr_render_async go (no counterpart in the
+ * reference, whose read_pixels copies the framebuffer, manager.py:228-230).
+ * HVR_READBACK_COPY: the device framebuffer, copied out by hvr_read (in
+ * chunks through pinned memory) or hvr_read_pinned.  HVR_READBACK_DIRECT
+ * (the default): the march writes each pixel straight into the context's
+ * pinned host buffer over PCIe as its ray finishes, so the image is on the
+ * host when the kernel ends; hvr_read then waits and copies it out once,
+ * hvr_read_pinned only waits.  Same bytes either way. */
+#define HVR_READBACK_COPY 0
+#define HVR_READBACK_DIRECT 1
+int hvr_set_readback(hvr_ctx *ctx, int mode);
 
 /* Samples composited per pixel for the current state (a separate counting
  * pass; not part of hvr_render).  `counts` is width*height uint32, bottom
  * row first. */
 int hvr_sample_counts(hvr_ctx *ctx, uint32_t *counts);
+/* The samples among those that load texels (no counterpart in the
+ * reference; for the texel-traffic account of a frame): hvr_sample_counts
+ * less the samples stepped over in transparent bricks when the frame's
+ * march skips them (hvr_set_empty_skip, hvr_skip_info). */
+int hvr_shaded_sample_counts(hvr_ctx *ctx, uint32_t *counts);
 
 /* Milliseconds of the last frame's march kernel and ray-setup kernel,
  * measured with HIP events on the context's stream. */
@@ -265,6 +287,21 @@ int hvr_set_empty_skip(hvr_ctx *ctx, int mode);
  * (0 and 0 when none were), and whether the last frame's march stepped over
  * them. */
 int hvr_skip_info(hvr_ctx *ctx, unsigned long long *clear_bricks, unsigned long long *bricks, int *used);
+
+/* Cost-ordered dispatch of single frames (no counterpart in the reference).
+ * A frame ends with its longest waves; in row order, long 16x16 ray blocks
+ * that start after the first round of resident blocks set its end.  With
+ * ordering, ray setup files every block under its rays' largest iteration
+ * bound and the march starts the longest first.  The image and the sample
+ * counts are identical in every mode.  HVR_ORDER_AUTO (the default) orders
+ * frames of more blocks than the device keeps resident at once (8 per
+ * compute unit) and at least 8 block rows. */
+#define HVR_ORDER_OFF 0
+#define HVR_ORDER_ON 1
+#define HVR_ORDER_AUTO 2
+int hvr_set_dispatch_order(hvr_ctx *ctx, int mode);
+/* Whether the last frame was cost-ordered. */
+int hvr_dispatch_order_info(hvr_ctx *ctx, int *used);
 
 /* Tuning knobs (no counterpart in the reference): log2 of the brick edge
  * used by volumes created afterwards: 2, 3 or 4 (4^3, 8^3 or 16^3 bricks,

@@ -20,7 +20,7 @@ _CSRC = os.path.join(_PKG, "csrc")
 HOST_SOURCE = os.path.join(_CSRC, "hipvr.hip")
 INST_SOURCE = os.path.join(_CSRC, "inst.hip")
 SOURCES = [HOST_SOURCE, INST_SOURCE]
-DEPENDS = SOURCES + [os.path.join(_CSRC, n) for n in ("hvr_device.h", "hvr_rules.h", "hvr_launch.h")] + \
+DEPENDS = SOURCES + [os.path.join(_CSRC, n) for n in ("hvr_device.h", "hvr_rules.h", "hvr_launch.h", "hvr_copy.h")] + \
     [os.path.join(os.path.dirname(_PKG), "include", "hipvr.h")]
 OUTPUT = os.path.join(_PKG, "libhipvr.so")
 OBJ_DIR = os.path.join(_PKG, "_build")

@@ -2,6 +2,7 @@
 // of hvr_device.h.  Host side: contexts (framebuffer, rays, LUT, stream,
 // events), volumes (device texels in brick layout), FP64 uniform
 // preparation and kernel dispatch.
+#include "hvr_copy.h"
 #include "hvr_launch.h"
 #include "../../include/hipvr.h"
 
@@ -89,6 +90,9 @@ struct hvr_ctx {
     int lut_n;       // entries in d_lut (>= 1)
     int lut_cap;
     void *h_pinned;  // W*H*4 readback staging
+    void *h_pinned_dev;   // its device address (the march's target in HVR_READBACK_DIRECT)
+    int readback_mode;    // HVR_READBACK_COPY / _DIRECT
+    bool last_host;       // the last frame was written to h_pinned
     float *d_stage;  // upload staging
     size_t stage_bytes;
     hipEvent_t ev[4];
@@ -109,6 +113,9 @@ struct hvr_ctx {
     uint32_t lut_gen;
     uint8_t *d_flags;      // per-brick transparent flags of flags_vol
     uint32_t *d_nclear;    // transparent-brick count written by k_brick_flags
+    uint32_t *h_nclear;    // its copy in pinned host memory, valid once flags_ev completed
+    hipEvent_t flags_ev;   // recorded after that copy
+    bool flags_count_known;   // flags_clear holds the count of the current flags
     uint64_t flags_cap;
     const hvr_volume *flags_vol;
     uint32_t flags_vol_gen, flags_lut_gen;
@@ -119,11 +126,29 @@ struct hvr_ctx {
     int ncu;               // compute units of the device
     size_t pix_cap;        // pixels d_out, d_rays and h_pinned hold
     size_t counts_cap;     // pixels d_counts holds
+    // cost-ordered dispatch of single frames (k_ray_setup_ordered)
+    uint32_t *d_order;     // two sets of 8 x kOrderBuckets counters, then the slots
+    size_t order_words;    // words d_order holds
+    int order_mode;        // HVR_ORDER_OFF / _ON / _AUTO
+    int order_used;        // whether the last frame was cost-ordered
+    int order_parity;      // counter set of the next ordered frame
+    // LUT uploads (hvr_set_lut): pinned staging slots holding the entries and
+    // their prefix counts, each reused once the copy that last read it is done
+    void *h_lut_stage[4];
+    hipEvent_t lut_ev[4];
+    bool lut_ev_live[4];
+    size_t lut_stage_bytes;   // bytes each slot holds
+    int lut_slot;             // the next slot
 };
 
 // HVR_SKIP_AUTO steps over transparent bricks when at least this fraction of
 // the bound volume's bricks are transparent for the current LUT.
 static constexpr double kSkipAutoFraction = 0.25;
+
+// HVR_ORDER_AUTO orders a frame's blocks by cost when they are more than the
+// march keeps resident at once (8 blocks of 256 threads per compute unit):
+// only then do blocks start late.
+static constexpr int kOrderAutoBlocksPerCU = 8;
 
 
 // ---------------------------------------------------------------------------
@@ -213,14 +238,32 @@ static void launch_march_generic(const MarchArgs &a)
 {
     const size_t lds = NM == NM_ANY ? 0 : 2 * (size_t)a.lut_n * sizeof(float4);
     hipLaunchKernelGGL((k_march<SF, NM, 3, COUNT, false, false>), dim3(a.nblocks), dim3(256), lds, a.stream, a.fc,
-                       a.ts, a.tn, a.lut, a.lut_n, a.rays, a.out, a.counts, a.skip, a.ntx, a.nblocks);
+                       a.ts, a.tn, a.lut, a.lut_n, a.rays, a.out, a.counts, a.skip, a.ntx, a.nblocks,
+                       a.order_cnt, a.order_slots, a.order_cap);
 }
 
-// Transparent-brick flags of volume v for the current LUT, recomputed (and
-// their count read back, which waits for the stream) when either changed.
+// The transparent-brick count of the current flags, once its copy to the
+// host has completed (waiting for it when `wait`).  Returns whether
+// flags_clear holds it.
+static bool flags_count(hvr_ctx *c, bool wait)
+{
+    if (c->flags_count_known) return true;
+    const hipError_t q = wait ? hipEventSynchronize(c->flags_ev) : hipEventQuery(c->flags_ev);
+    if (q == hipSuccess) {
+        c->flags_clear = *c->h_nclear;
+        c->flags_count_known = true;
+    } else if (q == hipErrorNotReady) {
+        (void)hipGetLastError();   // still in flight, not an error
+    }
+    return c->flags_count_known;
+}
+
+// Transparent-brick flags of volume v for the current LUT, recomputed when
+// either changed; their count comes back to the host without a wait.
 // Returns the flags when the march should step over transparent bricks:
-// always in HVR_SKIP_ON, in HVR_SKIP_AUTO when enough bricks are
-// transparent; nullptr otherwise.
+// always in HVR_SKIP_ON; in HVR_SKIP_AUTO when enough bricks are
+// transparent, or while the count is not back yet (skipping gives the same
+// image, so the choice only affects speed); nullptr otherwise.
 static const uint8_t *brick_flags(hvr_ctx *c, const hvr_volume *v)
 {
     if (c->skip_mode == HVR_SKIP_OFF || !v || !v->d_range || !c->d_nz) return nullptr;
@@ -236,24 +279,25 @@ static const uint8_t *brick_flags(hvr_ctx *c, const hvr_volume *v)
             if (hipMalloc(&c->d_flags, nb) != hipSuccess) return nullptr;
             c->flags_cap = nb;
         }
-        if (hipMemsetAsync(c->d_nclear, 0, sizeof(uint32_t), c->stream) != hipSuccess) return nullptr;
+        // the count comes back asynchronously (no wait in the frame's path)
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, c->d_nclear, 1, 0u);
         hipLaunchKernelGGL(k_brick_flags, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, c->stream,
                            (const float2 *)v->d_range, (uint32_t)nb, (const int *)c->d_nz, c->lut_n, c->d_flags,
                            c->d_nclear);
-        uint32_t clear = 0;
         if (hipGetLastError() != hipSuccess ||
-            hipMemcpyAsync(&clear, c->d_nclear, sizeof(clear), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-            hipStreamSynchronize(c->stream) != hipSuccess)
+            hipMemcpyAsync(c->h_nclear, c->d_nclear, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream) !=
+                hipSuccess ||
+            hipEventRecord(c->flags_ev, c->stream) != hipSuccess)
             return nullptr;
         c->flags_vol = v;
         c->flags_vol_gen = v->gen;
         c->flags_lut_gen = c->lut_gen;
-        c->flags_clear = clear;
+        c->flags_count_known = false;
         c->flags_total = nb;
     }
-    if (c->skip_mode == HVR_SKIP_ON || (double)c->flags_clear >= kSkipAutoFraction * (double)c->flags_total)
-        return c->d_flags;
-    return nullptr;
+    if (c->skip_mode == HVR_SKIP_ON) return c->d_flags;
+    if (!flags_count(c, false)) return c->d_flags;   // not back yet: skipping is exact either way
+    return (double)c->flags_clear >= kSkipAutoFraction * (double)c->flags_total ? c->d_flags : nullptr;
 }
 
 // The march of one frame into `out` (W*H RGBA8 on the device): the fast
@@ -261,10 +305,14 @@ static const uint8_t *brick_flags(hvr_ctx *c, const hvr_volume *v)
 // (rho, 0, 0) of the same volume, or unbound; brick edge 8 or 16), with
 // transparent-brick skipping when the flags say so and buffer loads below
 // 4 GiB, else the generic instance.
-static void launch_march(hvr_ctx *c, const FrameConst &fc, uchar4 *out, bool count)
+static void launch_march(hvr_ctx *c, const FrameConst &fc, uchar4 *out, bool count, const uint32_t *order_cnt,
+                         const uint32_t *order_slots, int order_cap)
 {
     MarchArgs a;
     a.fc = fc;
+    a.order_cnt = order_cnt;
+    a.order_slots = order_slots;
+    a.order_cap = order_cap;
     a.ntx = (c->W + 15) / 16;
     a.nblocks = a.ntx * ((c->H + 15) / 16);
     const hvr_volume *S = c->scalar_vol, *N = c->normal_vol;
@@ -317,9 +365,10 @@ static void launch_march(hvr_ctx *c, const FrameConst &fc, uchar4 *out, bool cou
     else launch_march_generic<TF_NONE, NM_ANY, false>(a);
 }
 
-static int enqueue_frame(hvr_ctx *c, bool count, uchar4 *dst)
+static int enqueue_frame(hvr_ctx *c, bool count, uchar4 *dst, bool count_shaded = false)
 {
-    const FrameConst fc = frame_const(c);
+    FrameConst fc = frame_const(c);
+    fc.count_shaded = count_shaded;
     const int npix = c->W * c->H;
     // events of this frame: the context's own, or the next slots of an
     // open timing region; hvr_last_kernel_ms reads the frame's events
@@ -330,17 +379,56 @@ static int enqueue_frame(hvr_ctx *c, bool count, uchar4 *dst)
         e1 = (*c->tev)[k + 1];
         e2 = (*c->tev)[k + 2];
     }
+    // cost-ordered dispatch (hvr_set_dispatch_order): the blocks of the
+    // first multiple-of-8 block rows are filed by ray setup and marched in
+    // decreasing order of their rays' iteration bounds
+    const int ntx = (c->W + 15) / 16, nty = (c->H + 15) / 16, nblocks = ntx * nty;
+    const int head_rows = nty >> 3 << 3, cap = (head_rows >> 3) * ntx;
+    const bool ordered = head_rows > 0 && (c->order_mode == HVR_ORDER_ON ||
+                                           (c->order_mode == HVR_ORDER_AUTO &&
+                                            nblocks > kOrderAutoBlocksPerCU * c->ncu));
+    constexpr int kSet = 8 * kOrderBuckets;   // counters per set
+    if (ordered) {
+        const size_t words = 2 * (size_t)kSet + (size_t)kSet * (size_t)cap;
+        if (words > c->order_words) {
+            HIPCHK(hipStreamSynchronize(c->stream));
+            (void)hipFree(c->d_order);
+            c->d_order = nullptr;
+            c->order_words = 0;
+            HIPCHK(hipMalloc(&c->d_order, words * sizeof(uint32_t)));
+            c->order_words = words;
+            // both counter sets start at zero (a kernel: no memset node in
+            // a graph a caller captures)
+            hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, c->d_order, 2 * kSet, 0u);
+            HIPCHK(hipGetLastError());
+            c->order_parity = 0;
+        }
+    }
+    c->order_used = ordered;
+    uint32_t *cnt = nullptr, *slots = nullptr;
     HIPCHK(hipEventRecord(e0, c->stream));
-    hipLaunchKernelGGL(k_ray_setup, dim3((npix + 255) / 256), dim3(256), 0, c->stream, fc, c->d_rays);
+    if (ordered) {
+        // this frame files into one counter set and clears the other
+        const int p = c->order_parity;
+        c->order_parity = p ^ 1;
+        cnt = c->d_order + p * kSet;
+        slots = c->d_order + 2 * kSet;
+        hipLaunchKernelGGL(k_ray_setup_ordered, dim3(nblocks), dim3(256), 0, c->stream, fc, c->d_rays, cnt,
+                           c->d_order + (p ^ 1) * kSet, slots, ntx, head_rows, cap);
+    } else {
+        hipLaunchKernelGGL(k_ray_setup, dim3((npix + 255) / 256), dim3(256), 0, c->stream, fc, c->d_rays);
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(e1, c->stream));
-    launch_march(c, fc, dst ? dst : c->d_out, count);
+    const bool host = !dst && c->readback_mode == HVR_READBACK_DIRECT;
+    launch_march(c, fc, dst ? dst : host ? (uchar4 *)c->h_pinned_dev : c->d_out, count, cnt, slots, cap);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(e2, c->stream));
     c->last_ev[0] = e0;
     c->last_ev[1] = e1;
     c->last_ev[2] = e2;
     c->events_recorded = true;
+    if (!dst) c->last_host = host;
     return HVR_OK;
 }
 
@@ -443,6 +531,8 @@ int hvr_create(int device, int width, int height, hvr_ctx **out)
     c->H = height;
     c->brick_log2 = 3;
     c->skip_mode = HVR_SKIP_AUTO;
+    c->order_mode = HVR_ORDER_AUTO;
+    c->readback_mode = HVR_READBACK_DIRECT;
     const char *gather = getenv("HVR_GATHER");
     c->force_global = gather && strcmp(gather, "global") == 0;
     const size_t npix = (size_t)width * height;
@@ -452,6 +542,10 @@ int hvr_create(int device, int width, int height, hvr_ctx **out)
     if (e == hipSuccess) e = hipMalloc(&c->d_out, npix * sizeof(uchar4));
     if (e == hipSuccess) e = hipMalloc(&c->d_rays, npix * sizeof(RayRec));
     if (e == hipSuccess) e = hipHostMalloc(&c->h_pinned, npix * 4, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostGetDevicePointer(&c->h_pinned_dev, c->h_pinned, 0);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_nclear, sizeof(uint32_t), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->flags_ev, hipEventDisableTiming);
+    for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&c->lut_ev[i], hipEventDisableTiming);
     for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
     for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&c->rev[i], hipEventDisableTiming);
     if (e == hipSuccess) {
@@ -479,19 +573,24 @@ int hvr_resize(hvr_ctx *c, int width, int height)
     if (width <= 0 || height <= 0 || (long long)width * height > (1ll << 28))
         return fail(HVR_EINVAL, "width and height must be positive (at most 2^28 pixels)");
     HIPCHK(hipSetDevice(c->device));
-    HIPCHK(hipStreamSynchronize(c->stream));
     const size_t npix = (size_t)width * height;
+    // Frames in flight keep their own sizes (kernel arguments); only
+    // buffers that must grow wait for them.
     if (npix > c->pix_cap) {
+        HIPCHK(hipStreamSynchronize(c->stream));
         (void)hipFree(c->d_out);
         (void)hipFree(c->d_rays);
         if (c->h_pinned) (void)hipHostFree(c->h_pinned);
         c->d_out = nullptr;
         c->d_rays = nullptr;
         c->h_pinned = nullptr;
+        c->h_pinned_dev = nullptr;
+        c->last_host = false;
         c->pix_cap = 0;
         HIPCHK(hipMalloc(&c->d_out, npix * sizeof(uchar4)));
         HIPCHK(hipMalloc(&c->d_rays, npix * sizeof(RayRec)));
         HIPCHK(hipHostMalloc(&c->h_pinned, npix * 4, hipHostMallocDefault));
+        HIPCHK(hipHostGetDevicePointer(&c->h_pinned_dev, c->h_pinned, 0));
         HIPCHK(hipMemset(c->d_out, 0, npix * sizeof(uchar4)));
         c->pix_cap = npix;
     }
@@ -513,10 +612,15 @@ int hvr_destroy(hvr_ctx *c)
     (void)hipFree(c->d_nz);
     (void)hipFree(c->d_flags);
     (void)hipFree(c->d_nclear);
+    (void)hipFree(c->d_order);
     if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+    if (c->h_nclear) (void)hipHostFree(c->h_nclear);
+    if (c->flags_ev) (void)hipEventDestroy(c->flags_ev);
     for (int i = 0; i < 4; ++i) {
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
         if (c->rev[i]) (void)hipEventDestroy(c->rev[i]);
+        if (c->lut_ev[i]) (void)hipEventDestroy(c->lut_ev[i]);
+        if (c->h_lut_stage[i]) (void)hipHostFree(c->h_lut_stage[i]);
     }
     if (c->tev) {
         for (hipEvent_t e : *c->tev) (void)hipEventDestroy(e);
@@ -536,10 +640,36 @@ int hvr_set_empty_skip(hvr_ctx *c, int mode)
     return HVR_OK;
 }
 
+int hvr_set_readback(hvr_ctx *c, int mode)
+{
+    if (!c) return fail(HVR_EINVAL, "ctx is null");
+    if (mode != HVR_READBACK_COPY && mode != HVR_READBACK_DIRECT)
+        return fail(HVR_EINVAL, "readback mode must be HVR_READBACK_COPY or HVR_READBACK_DIRECT");
+    c->readback_mode = mode;
+    return HVR_OK;
+}
+
+int hvr_set_dispatch_order(hvr_ctx *c, int mode)
+{
+    if (!c) return fail(HVR_EINVAL, "ctx is null");
+    if (mode != HVR_ORDER_OFF && mode != HVR_ORDER_ON && mode != HVR_ORDER_AUTO)
+        return fail(HVR_EINVAL, "dispatch order mode must be HVR_ORDER_OFF, HVR_ORDER_ON or HVR_ORDER_AUTO");
+    c->order_mode = mode;
+    return HVR_OK;
+}
+
+int hvr_dispatch_order_info(hvr_ctx *c, int *used)
+{
+    if (!c) return fail(HVR_EINVAL, "ctx is null");
+    if (used) *used = c->order_used;
+    return HVR_OK;
+}
+
 int hvr_skip_info(hvr_ctx *c, unsigned long long *clear_bricks, unsigned long long *bricks, int *used)
 {
     if (!c) return fail(HVR_EINVAL, "ctx is null");
     const bool valid = c->flags_vol != nullptr;
+    if (valid && !flags_count(c, true)) return fail(HVR_EDEVICE, "hvr_skip_info: reading the brick count failed");
     if (clear_bricks) *clear_bricks = valid ? c->flags_clear : 0;
     if (bricks) *bricks = valid ? c->flags_total : 0;
     if (used) *used = c->skip_used;
@@ -840,28 +970,51 @@ int hvr_set_lut(hvr_ctx *c, const float *rgba, int n)
     if (!c) return fail(HVR_EINVAL, "ctx is null");
     if (n < 0 || (n > 0 && !rgba)) return fail(HVR_EINVAL, "bad LUT");
     HIPCHK(hipSetDevice(c->device));
-    HIPCHK(hipStreamSynchronize(c->stream));
     const float unbound[4] = {0.f, 0.f, 0.f, 1.f};
     const float *src = n > 0 ? rgba : unbound;
     const int cnt = n > 0 ? n : 1;
+    // The copies are ordered on the stream after the frames already queued
+    // (which read the previous LUT), through pinned staging, so a LUT change
+    // does not wait for them; growing a buffer does.
+    const size_t lut_bytes = (size_t)cnt * sizeof(float4), nz_bytes = ((size_t)cnt + 1) * sizeof(int);
+    if (cnt > c->lut_cap || cnt > c->nz_cap || lut_bytes + nz_bytes > c->lut_stage_bytes)
+        HIPCHK(hipStreamSynchronize(c->stream));
     if (cnt > c->lut_cap) {
         (void)hipFree(c->d_lut);
         c->d_lut = nullptr;
         c->lut_cap = 0;
-        HIPCHK(hipMalloc(&c->d_lut, (size_t)cnt * sizeof(float4)));
+        HIPCHK(hipMalloc(&c->d_lut, lut_bytes));
         c->lut_cap = cnt;
     }
-    HIPCHK(hipMemcpy(c->d_lut, src, (size_t)cnt * sizeof(float4), hipMemcpyHostToDevice));
-    std::vector<int> nz((size_t)cnt + 1, 0);
-    lut_prefix_counts(src, cnt, nz.data());
     if (cnt > c->nz_cap) {
         (void)hipFree(c->d_nz);
         c->d_nz = nullptr;
         c->nz_cap = 0;
-        HIPCHK(hipMalloc(&c->d_nz, nz.size() * sizeof(int)));
+        HIPCHK(hipMalloc(&c->d_nz, nz_bytes));
         c->nz_cap = cnt;
     }
-    HIPCHK(hipMemcpy(c->d_nz, nz.data(), nz.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (lut_bytes + nz_bytes > c->lut_stage_bytes) {
+        size_t want = 4096;
+        while (want < lut_bytes + nz_bytes) want *= 2;
+        for (int i = 0; i < 4; ++i) {
+            if (c->h_lut_stage[i]) (void)hipHostFree(c->h_lut_stage[i]);
+            c->h_lut_stage[i] = nullptr;
+            c->lut_ev_live[i] = false;
+        }
+        c->lut_stage_bytes = 0;
+        for (int i = 0; i < 4; ++i) HIPCHK(hipHostMalloc(&c->h_lut_stage[i], want, hipHostMallocDefault));
+        c->lut_stage_bytes = want;
+    }
+    const int k = c->lut_slot;
+    c->lut_slot = (k + 1) & 3;
+    if (c->lut_ev_live[k]) HIPCHK(hipEventSynchronize(c->lut_ev[k]));
+    char *stage = (char *)c->h_lut_stage[k];
+    memcpy(stage, src, lut_bytes);
+    lut_prefix_counts(src, cnt, (int *)(stage + lut_bytes));
+    HIPCHK(hipMemcpyAsync(c->d_lut, stage, lut_bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_nz, stage + lut_bytes, nz_bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->lut_ev[k], c->stream));
+    c->lut_ev_live[k] = true;
     c->lut_n = cnt;
     ++c->lut_gen;
     return HVR_OK;
@@ -1006,12 +1159,19 @@ int hvr_timing_end(hvr_ctx *c, double *march_ms, double *setup_ms, int *frames)
 // device-to-host copy has landed, overlapping the transfer of the next.
 // Images up to 1 MiB take one copy, since the per-chunk synchronisation
 // costs more than the overlap saves there.
+
 int hvr_read(hvr_ctx *c, uint8_t *out)
 {
     HVR_FAULT_POINT("hvr_read");
     if (!c || !out) return fail(HVR_EINVAL, "null argument");
     HIPCHK(hipSetDevice(c->device));
     const size_t bytes = (size_t)c->W * c->H * 4;
+    if (c->last_host) {
+        // the march wrote the frame to the pinned buffer: wait, copy once
+        HIPCHK(hipStreamSynchronize(c->stream));
+        hvr::CopyPool::get().copy(out, (const uint8_t *)c->h_pinned, bytes);
+        return HVR_OK;
+    }
     constexpr int KMAX = 4;
     const int K = (int)std::min<size_t>(KMAX, std::max<size_t>(1, bytes >> 20));
     const size_t chunk = ((bytes + K - 1) / K + 4095) & ~(size_t)4095;
@@ -1037,7 +1197,7 @@ int hvr_read_pinned(hvr_ctx *c, const uint8_t **rgba)
     *rgba = nullptr;
     HIPCHK(hipSetDevice(c->device));
     const size_t bytes = (size_t)c->W * c->H * 4;
-    HIPCHK(hipMemcpyAsync(c->h_pinned, c->d_out, bytes, hipMemcpyDeviceToHost, c->stream));
+    if (!c->last_host) HIPCHK(hipMemcpyAsync(c->h_pinned, c->d_out, bytes, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     *rgba = (const uint8_t *)c->h_pinned;
     return HVR_OK;
@@ -1065,9 +1225,8 @@ int hvr_framebuffer(hvr_ctx *c, void **ptr)
     return HVR_OK;
 }
 
-int hvr_sample_counts(hvr_ctx *c, uint32_t *counts)
+static int sample_counts(hvr_ctx *c, uint32_t *counts, bool shaded)
 {
-    HVR_FAULT_POINT("hvr_sample_counts");
     if (!c || !counts) return fail(HVR_EINVAL, "null argument");
     HIPCHK(hipSetDevice(c->device));
     const size_t npix = (size_t)c->W * c->H;
@@ -1078,11 +1237,22 @@ int hvr_sample_counts(hvr_ctx *c, uint32_t *counts)
         HIPCHK(hipMalloc(&c->d_counts, npix * sizeof(uint32_t)));
         c->counts_cap = npix;
     }
-    int rc = enqueue_frame(c, true, nullptr);
+    int rc = enqueue_frame(c, true, nullptr, shaded);
     if (rc != HVR_OK) return rc;
     HIPCHK(hipMemcpyAsync(counts, c->d_counts, npix * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return HVR_OK;
+}
+
+int hvr_sample_counts(hvr_ctx *c, uint32_t *counts)
+{
+    HVR_FAULT_POINT("hvr_sample_counts");
+    return sample_counts(c, counts, false);
+}
+
+int hvr_shaded_sample_counts(hvr_ctx *c, uint32_t *counts)
+{
+    return sample_counts(c, counts, true);
 }
 
 int hvr_last_kernel_ms(hvr_ctx *c, float *march_ms, float *setup_ms)

@@ -551,6 +551,7 @@ struct FrameConst {
     int L_ok;         // 0: zero-length light vector -> diffuse term 0
     uint32_t clear[4];  // framebuffer clear colour (8-bit per channel)
     int blend;          // 1: SRC_ALPHA, ONE_MINUS_SRC_ALPHA over the clear colour
+    int count_shaded;   // counting instances: count only samples that load texels
 };
 
 // One ray per pixel, produced by the setup kernel.
@@ -620,6 +621,41 @@ static __global__ void __launch_bounds__(256) k_ray_setup(const FrameConst fc, R
     const int idx = blockIdx.x * 256 + threadIdx.x;
     if (idx >= fc.W * fc.H) return;
     rays[idx] = ray_setup(fc, idx % fc.W, idx / fc.W);
+}
+
+// K1 for a cost-ordered frame: one block per 16x16 ray block (row-major,
+// ntx per row), which also files the block for the march's dispatch order
+// (ordered_block): its key is the largest iteration bound of its rays.
+// `cnt` holds 8 x kOrderBuckets counters (zero on entry), `slots` cap
+// entries per class and bucket; only the blocks of the first `head_rows`
+// block rows (a multiple of 8) are filed.  Block 0 also clears `clear`, the
+// other set of counters, which the previous ordered frame's march read and
+// the next ordered frame fills (counters alternate between frames, so no
+// separate clearing launch is needed).
+static __global__ void __launch_bounds__(256)
+k_ray_setup_ordered(const FrameConst fc, RayRec *__restrict__ rays, uint32_t *__restrict__ cnt,
+                    uint32_t *__restrict__ clear, uint32_t *__restrict__ slots, int ntx, int head_rows, int cap)
+{
+    __shared__ int s_key[4];
+    const int t = blockIdx.x, tx = t % ntx, ty = t / ntx;
+    if (t == 0 && threadIdx.x < 8 * kOrderBuckets) clear[threadIdx.x] = 0u;
+    const int px = tx * 16 + (threadIdx.x & 15), py = ty * 16 + (threadIdx.x >> 4);
+    int key = 0;
+    if (px < fc.W && py < fc.H) {
+        const RayRec r = ray_setup(fc, px, py);
+        rays[py * fc.W + px] = r;
+        key = r.nlim;
+    }
+    if (ty >= head_rows) return;   // the whole block
+    for (int off = 32; off > 0; off >>= 1) key = max(key, __shfl_xor(key, off, 64));
+    if ((threadIdx.x & 63) == 0) s_key[threadIdx.x >> 6] = key;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int m = max(max(s_key[0], s_key[1]), max(s_key[2], s_key[3]));
+        const int q = (ty & 7) * kOrderBuckets + order_bucket(m, fc.max_steps);
+        const uint32_t s = atomicAdd(cnt + q, 1u);
+        if (s < (uint32_t)cap) slots[(uint64_t)q * (uint64_t)cap + s] = (uint32_t)t;
+    }
 }
 
 // Stored RGBA8 (manager.py:29-31, 212-220): the source colour is clamped
@@ -725,7 +761,11 @@ __device__ __forceinline__ void composite(const FrameConst &fc, const float4 &q,
 // strongly over the image (where the rays cross dense matter), so
 // contiguous bands per XCD left some XCDs idle for half the frame, while
 // interleaved rows give every XCD an even share, and its L2 still sees whole
-// rows.  Rows past the last multiple of 8 keep the plain order.  A wave runs
+// rows.  Rows past the last multiple of 8 keep the plain order.  With
+// `order` (a cost-ordered frame, k_ray_setup_ordered), each XCD's class of
+// rows is taken in decreasing order of its blocks' iteration bounds
+// instead (ordered_block), so that the long blocks start in the first round
+// of resident blocks rather than after it.  A wave runs
 // while any of its rays is live; a ray stops at its iteration bound
 // (max_steps at most), when its alpha reaches tau, or when it leaves the box.
 //
@@ -882,7 +922,7 @@ __device__ __forceinline__ void march_ray(const FrameConst &fc, const TexDesc &t
             while (in && A < fc.tau) {
                 if (!clear) shade(f);
                 ++i;
-                if constexpr (COUNT) ++ns;
+                if constexpr (COUNT) ns += (fc.count_shaded && clear) ? 0u : 1u;
                 x = __fadd_rn(x, dqx);
                 y = __fadd_rn(y, dqy);
                 z = __fadd_rn(z, dqz);
@@ -921,7 +961,8 @@ template <int SF, int NM, int LB, bool COUNT, bool SKIP, bool BUF>
 __global__ void __launch_bounds__(256)
 k_march(const FrameConst fc, const TexDesc ts, const TexDesc tn, const float4 *__restrict__ lut_g,
         int lut_n, const RayRec *__restrict__ rays, uchar4 *__restrict__ out,
-        uint32_t *__restrict__ counts, const uint8_t *__restrict__ skip, int ntx, int nblocks)
+        uint32_t *__restrict__ counts, const uint8_t *__restrict__ skip, int ntx, int nblocks,
+        const uint32_t *__restrict__ order_cnt, const uint32_t *__restrict__ order_slots, int order_cap)
 {
     extern __shared__ float4 s_lut[];
     if constexpr (NM != NM_ANY) {
@@ -931,7 +972,11 @@ k_march(const FrameConst fc, const TexDesc ts, const TexDesc tn, const float4 *_
     const int b = blockIdx.x;
     const int head = ((nblocks / ntx) >> 3 << 3) * ntx;   // blocks of the first multiple-of-8 rows
     int logical = b;
-    if (b < head) {
+    if (order_cnt) {
+        // cost-ordered (k_ray_setup_ordered filed the blocks)
+        logical = ordered_block(b, head, order_cnt, order_slots, order_cap);
+        if (logical < 0 || logical >= nblocks) logical = b;
+    } else if (b < head) {
         const int j = b >> 3;   // j-th block of its XCD
         logical = ((j / ntx) * 8 + (b & 7)) * ntx + j % ntx;
     }

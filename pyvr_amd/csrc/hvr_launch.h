@@ -20,6 +20,9 @@ struct MarchArgs {
     uint32_t *counts;
     const uint8_t *skip;
     int ntx, nblocks;
+    const uint32_t *order_cnt;     // cost-ordered dispatch (k_ray_setup_ordered), or nullptr
+    const uint32_t *order_slots;
+    int order_cap;
     hipStream_t stream;
 };
 

@@ -90,6 +90,46 @@ constexpr void lut_prefix_counts(const float *src, int cnt, int *nz)
     }
 }
 
+// Cost-ordered dispatch of a frame's 16x16 ray blocks.  A frame ends when
+// its longest waves do; dispatched in row order, long blocks that start
+// late (after the first round of resident blocks) set the frame's end.
+// Ray setup gives each block a cost key, the largest iteration bound of its
+// rays, and files the block under a key bucket; the march then takes the
+// blocks of the largest keys first.  Buckets split [0, max_steps] evenly,
+// bucket 0 holding the largest keys.
+constexpr int kOrderBuckets = 16;
+constexpr int order_bucket(int key, int max_steps)
+{
+    if (key <= 0 || max_steps <= 0) return kOrderBuckets - 1;
+    const long long k = (long long)key * kOrderBuckets / ((long long)max_steps + 1);
+    return kOrderBuckets - 1 - (int)(k < kOrderBuckets - 1 ? k : kOrderBuckets - 1);
+}
+
+// Dispatch position b of a cost-ordered frame -> logical block (row-major,
+// `ntx` blocks per row).  The first `head` positions (the blocks of the
+// first multiple-of-8 block rows) are dealt to 8 classes in turn: class
+// b & 7 holds block rows class, class + 8, ... (blocks are dispatched
+// round-robin over the 8 XCDs, so one class runs on one XCD and its L2 sees
+// whole rows), and within a class the blocks are taken bucket by bucket
+// from `slots` (`cap` entries per class and bucket, cap = the class's
+// block count) in the numbers `cnt` holds (kOrderBuckets per class).
+// Later positions keep the row-major order.  Returns b itself when the
+// counts do not cover the position (never with counts ray setup wrote).
+template <typename U>
+constexpr int ordered_block(int b, int head, const U *cnt, const U *slots, int cap)
+{
+    if (b >= head) return b;
+    const int x = b & 7;
+    uint32_t j = (uint32_t)(b >> 3);
+    for (int k = 0; k < kOrderBuckets; ++k) {
+        const uint32_t c = (uint32_t)cnt[x * kOrderBuckets + k];
+        if (j < c)
+            return j < (uint32_t)cap ? (int)slots[(uint64_t)(x * kOrderBuckets + k) * (uint64_t)cap + j] : b;
+        j -= c;
+    }
+    return b;
+}
+
 // Inverse of a 4x4 matrix (row-major, m[4*r + c]) by Gauss-Jordan
 // elimination with partial pivoting.  Returns false when singular.
 constexpr bool inverse4(const double *m, double *inv)

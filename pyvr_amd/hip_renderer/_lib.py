@@ -43,7 +43,8 @@ EXPORTS = (
     "hvr_batch_counts", "hvr_batch_last_ms", "hvr_volume_upload_scalar_normals", "hvr_volume_generate_blobs",
     "hvr_batch_query", "hvr_device_memory", "hvr_read_pinned", "hvr_volume_download",
     "hvr_resize", "hvr_debug_inject_fault", "hvr_render_async_to_noalloc",
-    "hvr_batch_set_kernel",
+    "hvr_batch_set_kernel", "hvr_set_dispatch_order", "hvr_dispatch_order_info", "hvr_set_readback",
+    "hvr_shaded_sample_counts",
 )
 
 # batch ray-tile shapes (hvr_batch_set_tile_shape)
@@ -51,6 +52,12 @@ TILE_8X8, TILE_16X4 = 0, 1
 
 # transparent-brick skipping modes (hvr_set_empty_skip)
 SKIP_OFF, SKIP_ON, SKIP_AUTO = 0, 1, 2
+
+# cost-ordered dispatch of single frames (hvr_set_dispatch_order)
+ORDER_OFF, ORDER_ON, ORDER_AUTO = 0, 1, 2
+
+# where single frames go (hvr_set_readback)
+READBACK_COPY, READBACK_DIRECT = 0, 1
 
 
 class HvrParams(ctypes.Structure):
@@ -105,9 +112,13 @@ def _declare(L):
         "hvr_synchronize": ([vp], i),
         "hvr_framebuffer": ([vp, ctypes.POINTER(vp)], i),
         "hvr_sample_counts": ([vp, ctypes.POINTER(ctypes.c_uint32)], i),
+        "hvr_shaded_sample_counts": ([vp, ctypes.POINTER(ctypes.c_uint32)], i),
         "hvr_last_kernel_ms": ([vp, fp, fp], i),
         "hvr_set_brick_log2": ([vp, i], i),
         "hvr_set_empty_skip": ([vp, i], i),
+        "hvr_set_dispatch_order": ([vp, i], i),
+        "hvr_set_readback": ([vp, i], i),
+        "hvr_dispatch_order_info": ([vp, ctypes.POINTER(i)], i),
         "hvr_skip_info": ([vp, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_ulonglong),
                            ctypes.POINTER(i)], i),
         "hvr_batch_create": ([vp, ctypes.POINTER(vp)], i),

@@ -141,6 +141,15 @@ class PlainRenderer:
               "hvr_sample_counts")
         return c.reshape(h, w)
 
+    def shaded_sample_counts(self) -> np.ndarray:
+        """Of those, the samples that load texels (not stepped over in
+        transparent bricks), per pixel (H, W)."""
+        w, h = self.size
+        c = np.empty(w * h, np.uint32)
+        check(self.L.hvr_shaded_sample_counts(self.ctx, c.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))),
+              "hvr_shaded_sample_counts")
+        return c.reshape(h, w)
+
     def download(self, volume, normals: bool = True):
         """The resident volume's stored fields (scalar as the march reads it,
         normals as stored) as float32 arrays, for checking on the CPU."""

@@ -10,7 +10,12 @@
 //    alpha 0 and finite colours through the march's LUT read
 //    (v[i] + a * (v[i+1] - v[i]) at u = clamp(rho*N - 1/2, 0, N-1));
 //  * lut_prefix_counts against a direct count;
-//  * inverse4: M * inverse4(M) = I on random matrices, singular rejected.
+//  * inverse4: M * inverse4(M) = I on random matrices, singular rejected;
+//  * cost-ordered dispatch: blocks filed as the ordered ray setup files them
+//    (in a random order, as blocks run) and read back through
+//    ordered_block give every block exactly once, keep each dispatch class
+//    on its own block rows (class = position & 7 -> rows = class mod 8), and
+//    take a class's blocks in non-decreasing bucket order (decreasing keys).
 // Exit status 0 on success.
 #include <math.h>
 #include <stdint.h>
@@ -144,8 +149,59 @@ static void check_inverse()
     CHECK(!inverse4(rank3, inv), "singular matrix inverted");
 }
 
+static void check_order(int ntx, int nty, int max_steps)
+{
+    const int nblocks = ntx * nty, head_rows = nty >> 3 << 3, cap = (head_rows >> 3) * ntx;
+    const int head = head_rows * ntx;
+    std::vector<uint32_t> cnt(8 * kOrderBuckets, 0u), slots((size_t)8 * kOrderBuckets * (cap > 0 ? cap : 1), ~0u);
+    std::vector<int> key(nblocks), perm(nblocks);
+    for (int t = 0; t < nblocks; ++t) {
+        key[t] = irand(4) == 0 ? 0 : irand(max_steps + 1);
+        perm[t] = t;
+    }
+    for (int t = nblocks - 1; t > 0; --t) {   // blocks file themselves in any order
+        const int u = irand(t + 1), x = perm[t];
+        perm[t] = perm[u];
+        perm[u] = x;
+    }
+    for (int t : perm) {
+        const int ty = t / ntx;
+        if (ty >= head_rows) continue;
+        const int q = (ty & 7) * kOrderBuckets + order_bucket(key[t], max_steps);
+        const uint32_t s = cnt[q]++;
+        CHECK(s < (uint32_t)cap, "ntx=%d nty=%d: class bucket over capacity", ntx, nty);
+        if (s < (uint32_t)cap) slots[(size_t)q * cap + s] = (uint32_t)t;
+    }
+    std::vector<int> seen(nblocks, 0);
+    std::vector<int> last_bucket(8, -1);
+    for (int b = 0; b < nblocks; ++b) {
+        const int l = ordered_block(b, head, cnt.data(), slots.data(), cap);
+        CHECK(l >= 0 && l < nblocks, "ntx=%d nty=%d b=%d: block %d out of range", ntx, nty, b, l);
+        if (l < 0 || l >= nblocks) continue;
+        ++seen[l];
+        if (b < head) {
+            CHECK((l / ntx) % 8 == (b & 7), "ntx=%d nty=%d b=%d: row %d not in class %d", ntx, nty, b, l / ntx, b & 7);
+            const int k = order_bucket(key[l], max_steps);
+            CHECK(k >= last_bucket[b & 7], "ntx=%d nty=%d b=%d: bucket order", ntx, nty, b);
+            last_bucket[b & 7] = k;
+        } else {
+            CHECK(l == b, "ntx=%d nty=%d b=%d: tail block moved", ntx, nty, b);
+        }
+    }
+    for (int t = 0; t < nblocks; ++t) CHECK(seen[t] == 1, "ntx=%d nty=%d: block %d taken %d times", ntx, nty, t, seen[t]);
+    // buckets: 0 holds the largest keys; keys <= 0 the last bucket
+    CHECK(order_bucket(0, max_steps) == kOrderBuckets - 1 &&
+              (max_steps < kOrderBuckets - 1 || order_bucket(max_steps, max_steps) == 0),
+          "bucket ends (max_steps %d)", max_steps);
+    for (int k = 1; k <= max_steps; k += 1 + max_steps / 97)
+        CHECK(order_bucket(k, max_steps) <= order_bucket(k - 1, max_steps), "bucket monotonicity at %d", k);
+}
+
 int main()
 {
+    for (int i = 0; i < 300; ++i) check_order(1 + irand(40), 1 + irand(40), 1 + irand(2000));
+    check_order(64, 64, 1000);
+    check_order(128, 128, 1000);
     for (int lb = 2; lb <= 4; ++lb)
         for (int s = 1; s <= 70; ++s) check_layout(s, lb);
     check_brick_rule();

@@ -226,3 +226,31 @@ def test_pinned_read_equals_copy_read(hip):
                 assert a == pinned == bytes(out) and max(a) > 0, (w, h, az)
         finally:
             r.close()
+
+
+def test_shaded_sample_counts(hip):
+    """hvr_shaded_sample_counts: the samples that load texels.  Without
+    skipping, every counted sample does; with skipping, those in transparent
+    bricks do not, so per pixel 0 <= shaded <= counted, with fewer in total
+    on a scene with empty space; the march itself is unchanged."""
+    import ctypes
+
+    from pyvr_amd.hip_renderer import _lib
+
+    sc = scenes.c2(size=96, image=128)
+    r = _prepared(sc)
+    L, ctx = _lib.lib(), r.gl_manager.ctx
+    try:
+        out = {}
+        for mode in (_lib.SKIP_OFF, _lib.SKIP_ON):
+            img, cnt, _ = _render_raw(r, sc.lut(), mode)
+            sh = np.zeros_like(cnt)
+            _lib.check(L.hvr_shaded_sample_counts(ctx, sh.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))), "shaded")
+            out[mode] = (img, cnt, sh)
+    finally:
+        r.close()
+    img0, cnt0, sh0 = out[_lib.SKIP_OFF]
+    img1, cnt1, sh1 = out[_lib.SKIP_ON]
+    print(f"[shaded] samples {int(cnt1.sum())}, shaded {int(sh1.sum())} with skipping", flush=True)
+    assert np.array_equal(sh0, cnt0) and np.array_equal(cnt0, cnt1) and np.array_equal(img0, img1)
+    assert (sh1 <= cnt1).all() and 0 < sh1.sum() < cnt1.sum()

@@ -30,8 +30,9 @@ def test_oracle_under_address_and_undefined_behaviour_sanitizers():
 
 @pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not available")
 def test_layout_and_brick_rules_under_sanitizers(tmp_path):
-    """The texel-layout, transparent-brick, LUT-prefix and matrix-inverse
-    rules the kernels and the host side use (pyvr_amd/csrc/hvr_rules.h),
+    """The texel-layout, transparent-brick, LUT-prefix, matrix-inverse and
+    cost-ordered-dispatch rules the kernels and the host side use
+    (pyvr_amd/csrc/hvr_rules.h),
     compiled natively with the sanitizers and brute-forced by
     tests/native/rules_check.cpp."""
     root = os.path.dirname(ORACLE)
@@ -45,3 +46,19 @@ def test_layout_and_brick_rules_under_sanitizers(tmp_path):
     res = subprocess.run([str(exe)], capture_output=True, text=True, env=env, timeout=300)
     assert res.returncode == 0, res.stdout + res.stderr
     assert "runtime error" not in res.stderr and "rules ok" in res.stdout
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not available")
+def test_frame_copy_pool_under_thread_sanitizer(tmp_path):
+    """The worker pool that copies whole frames out of pinned memory
+    (pyvr_amd/csrc/hvr_copy.h), compiled natively with ThreadSanitizer and
+    checked against memcpy by tests/native/copy_check.cpp."""
+    root = os.path.dirname(ORACLE)
+    exe = tmp_path / "copy_check"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-fsanitize=thread", "-Wall", "-Wextra", "-Werror",
+                    "-I", os.path.join(root, "pyvr_amd", "csrc"),
+                    os.path.join(root, "tests", "native", "copy_check.cpp"), "-o", str(exe), "-pthread"], check=True)
+    env = dict(os.environ, TSAN_OPTIONS="halt_on_error=1")
+    res = subprocess.run([str(exe)], capture_output=True, text=True, env=env, timeout=300)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "ThreadSanitizer" not in res.stderr and "copy ok" in res.stdout
