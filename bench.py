@@ -494,7 +494,11 @@ def serving_leg(args, rates, duration, nvol, plain_requests):
         lat = np.array([r.latency for r in done]) * 1e3
         end = max(r.done for r in done)
         tot = int(samples[:n].sum())
+        slow = sorted(done, key=lambda r: -r.latency)[:5]
         return {"rate": "saturation" if rate is None else rate, "requests": n,
+                "slowest": [{"arrival_s": r.arrival - start, "queue_ms": (r.launched - r.arrival) * 1e3,
+                             "run_ms": (r.done - r.launched) * 1e3, "size": [r.width, r.height],
+                             "step": float(r.config.step_size)} for r in slow],
                 "p50_ms": float(np.percentile(lat, 50)),
                 "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
                 "samples_per_s": tot / (end - start), "achieved_req_per_s": n / (end - start),
