@@ -154,7 +154,7 @@ def measured_profile(key: str) -> dict:
     return e
 
 
-def roofline_of(kernel, samples, bpv, out_bytes, ms, key, shaded_fraction=None):
+def roofline_of(kernel, samples, bpv, out_bytes, ms, key, shaded_fraction=None, floor_bytes=None):
     """Roofline object of one launch of ``kernel``: measured HBM bytes per
     launch over its duration against HBM peak (``frac``), the algorithmic
     bytes beside it (``alg``), and the VALU issue roof (``valu``) from the
@@ -180,6 +180,12 @@ def roofline_of(kernel, samples, bpv, out_bytes, ms, key, shaded_fraction=None):
         shaded = samples * shaded_fraction * bpv + out_bytes
         out["alg"].update({"shaded_fraction": shaded_fraction, "shaded_bytes_per_launch": shaded,
                            "shaded_achieved": shaded / t / 1e9, "shaded_frac": shaded / t / HBM_PEAK})
+    if floor_bytes is not None:
+        # compulsory traffic: the bricks the launch's samples read, once
+        # each per frame, and the images
+        out["floor"] = {"bytes_per_launch": floor_bytes, "achieved": floor_bytes / t / 1e9,
+                        "frac": floor_bytes / t / HBM_PEAK,
+                        "traffic_over_floor": None if traffic is None else traffic / floor_bytes}
     if "valu_per_sample" in prof:
         va = prof["valu_per_sample"] * samples / t
         out["valu"] = {"instr_per_sample": prof["valu_per_sample"], "achieved": va / 1e12, "peak": VALU_PEAK / 1e12,
@@ -267,15 +273,19 @@ def plain_leg(wl: Workload, args, sf, nf):
     pr.upload(wl.volume)
     vol_bytes = pr.resident_bytes()
     log(f"volume resident: {vol_bytes / 2**20:.0f} MiB in {time.perf_counter() - t0:.1f}s")
-    params, counts, shaded = [], [], []
+    params, counts, shaded, touched = [], [], [], []
     for cam in wl.cams:
         pr.prepare(wl.volume, cam, wl.ctf, wl.otf, wl.width, wl.height, wl.config, wl.light, wl.tau)
         params.append(pr.params)
         counts.append(int(pr.sample_counts().sum()))
         shaded.append(int(pr.shaded_sample_counts().sum()))
+        touched.append(pr.touched_bytes())
     s_step = sum(counts)
     shaded_fraction = sum(shaded) / max(1, s_step)
     npix = wl.width * wl.height
+    # compulsory floor: each frame reads the bricks its samples load from
+    # once, and writes its image
+    floor_frames = [t + 4 * npix for t in touched]
     log(f"samples per step: {s_step} ({len(wl.cams)} frames)")
     bufs = []
     for _ in wl.cams:
@@ -349,11 +359,12 @@ def plain_leg(wl: Workload, args, sf, nf):
     key = (f"{wl.name}/{len(wl.cams)}cams/{sf}+{nf if wl.has_normals else 'none'}/plain/lb{lb}"
            f"/skip{int(used.value)}/order{int(ordered.value)}")
     roofline = roofline_of("hvr::k_march", s_step / len(wl.cams), bpv, 4 * npix, march_ms.value / nfr.value, key,
-                           shaded_fraction)
+                           shaded_fraction, sum(floor_frames) / len(floor_frames))
     return {"value": value, "ms_per_step": ms_step, "samples": s_step, "march_kernel_ms": march_ms.value / nfr.value,
             "fps": 1e3 / float(lat.mean()), "frame_ms_p50": float(np.percentile(lat, 50)),
             "frame_ms_p99": float(np.percentile(lat, 99)), "frames_per_s_device": len(wl.cams) * args.steps / wall,
-            "roofline": roofline, "shaded_fraction": shaded_fraction, "brick_skip": skip, "dispatch_order": order, "volume_bytes": vol_bytes, "bpv": bpv}, stored
+            "roofline": roofline, "shaded_fraction": shaded_fraction, "floor_bytes_per_step": sum(floor_frames),
+            "brick_skip": skip, "dispatch_order": order, "volume_bytes": vol_bytes, "bpv": bpv}, stored
 
 
 def batched_leg(wl: Workload, args, sf, nf, s_step):
@@ -770,7 +781,7 @@ def main():
         sh = plain["shaded_fraction"] if batched["kernel"] != "noskip" else 1.0
         roofline = roofline_of("hvr::k_batch (graph launch: staging copy, counter fill, brick flags, k_batch)",
                                batched["samples"], bpv, 4 * npix * len(wl.cams), batched["device_ms_per_run"], bkey,
-                               sh)
+                               sh, plain["floor_bytes_per_step"])
     serving = None
     if args.workload == "c2" and not args.plain_only and not args.no_serve:
         serving = serving_leg(args, [float(r) for r in args.serve_rates.split(",")], args.serve_duration,

@@ -244,6 +244,13 @@ int hvr_sample_counts(hvr_ctx *ctx, uint32_t *counts);
  * less the samples stepped over in transparent bricks when the frame's
  * march skips them (hvr_set_empty_skip, hvr_skip_info). */
 int hvr_shaded_sample_counts(hvr_ctx *ctx, uint32_t *counts);
+/* The bricks the current frame's samples load texels from (no counterpart
+ * in the reference; the compulsory texel traffic of a frame is `touched` x
+ * `brick_bytes`): a counting pass that marks them, of `bricks` in the bound
+ * scalar volume.  Needs the fast path's conditions (8^3 or 16^3 bricks, a
+ * LUT of at most 2048 entries); waits. */
+int hvr_touched_bricks(hvr_ctx *ctx, unsigned long long *touched, unsigned long long *bricks,
+                       unsigned long long *brick_bytes);
 
 /* Milliseconds of the last frame's march kernel and ray-setup kernel,
  * measured with HIP events on the context's stream. */
@@ -345,6 +352,14 @@ int hvr_batch_add(hvr_batch *batch, hvr_volume *scalar_vol, int scalar_role, hvr
 /* Enqueue the whole batch on the batch's stream (count != 0: also record
  * per-pixel sample counts); the images are copied to pinned host memory.
  * hvr_batch_run waits for them. */
+/* Grow the batch's pools now to what a batch of up to `max_requests`
+ * requests, `max_pixels` pixels in all and volumes of up to `max_bricks`
+ * bricks needs (outputs, brick flags, staged block with LUTs of up to 2048
+ * entries), so that runs within those limits never allocate (growing a
+ * pool frees the old one, which waits for the device).  Waits for the
+ * batch's stream. */
+int hvr_batch_reserve(hvr_batch *batch, int max_requests, unsigned long long max_pixels,
+                      unsigned long long max_bricks);
 int hvr_batch_run_async(hvr_batch *batch, int count);
 int hvr_batch_wait(hvr_batch *batch);
 /* Non-blocking: *done = 1 when the last run's images (and counts) are in

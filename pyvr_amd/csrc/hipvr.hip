@@ -365,10 +365,12 @@ static void launch_march(hvr_ctx *c, const FrameConst &fc, uchar4 *out, bool cou
     else launch_march_generic<TF_NONE, NM_ANY, false>(a);
 }
 
-static int enqueue_frame(hvr_ctx *c, bool count, uchar4 *dst, bool count_shaded = false)
+static int enqueue_frame(hvr_ctx *c, bool count, uchar4 *dst, bool count_shaded = false,
+                         uint8_t *touched = nullptr)
 {
     FrameConst fc = frame_const(c);
     fc.count_shaded = count_shaded;
+    fc.touched = touched;
     const int npix = c->W * c->H;
     // events of this frame: the context's own, or the next slots of an
     // open timing region; hvr_last_kernel_ms reads the frame's events
@@ -1255,6 +1257,42 @@ int hvr_shaded_sample_counts(hvr_ctx *c, uint32_t *counts)
     return sample_counts(c, counts, true);
 }
 
+int hvr_touched_bricks(hvr_ctx *c, unsigned long long *touched, unsigned long long *bricks,
+                       unsigned long long *brick_bytes)
+{
+    if (!c || !touched) return fail(HVR_EINVAL, "null argument");
+    const hvr_volume *S = c->scalar_vol;
+    if (!S || !(S->lb == 3 || S->lb == 4) || c->lut_n > kLutLdsMax)
+        return fail(HVR_ESTATE, "hvr_touched_bricks: needs a bound scalar volume with 8^3 or 16^3 bricks and a LUT "
+                                "of at most 2048 entries");
+    HIPCHK(hipSetDevice(c->device));
+    const size_t npix = (size_t)c->W * c->H;
+    if (c->counts_cap < npix) {
+        (void)hipFree(c->d_counts);
+        c->d_counts = nullptr;
+        c->counts_cap = 0;
+        HIPCHK(hipMalloc(&c->d_counts, npix * sizeof(uint32_t)));
+        c->counts_cap = npix;
+    }
+    const uint64_t nb = (uint64_t)S->nbx * S->nby * S->nbz;
+    uint8_t *d_marks = nullptr;
+    HIPCHK(hipMalloc(&d_marks, nb));
+    std::vector<uint8_t> marks(nb);
+    int rc = hipMemsetAsync(d_marks, 0, nb, c->stream) == hipSuccess ? HVR_OK : fail(HVR_EDEVICE, "memset");
+    if (rc == HVR_OK) rc = enqueue_frame(c, true, nullptr, false, d_marks);
+    if (rc == HVR_OK && (hipMemcpyAsync(marks.data(), d_marks, nb, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+                         hipStreamSynchronize(c->stream) != hipSuccess))
+        rc = fail(HVR_EDEVICE, "hvr_touched_bricks: reading the marks failed");
+    (void)hipFree(d_marks);
+    if (rc != HVR_OK) return rc;
+    unsigned long long n = 0;
+    for (uint8_t m : marks) n += m != 0;
+    *touched = n;
+    if (bricks) *bricks = nb;
+    if (brick_bytes) *brick_bytes = S->brick_bytes;
+    return HVR_OK;
+}
+
 int hvr_last_kernel_ms(hvr_ctx *c, float *march_ms, float *setup_ms)
 {
     if (!c) return fail(HVR_EINVAL, "ctx is null");
@@ -1480,6 +1518,29 @@ int hvr_batch_add(hvr_batch *b, hvr_volume *s, int srole, hvr_volume *n, int nro
     b->reqs.push_back(std::move(r));
     b->ran = b->counted = false;
     if (index) *index = (int)b->reqs.size() - 1;
+    return HVR_OK;
+}
+
+int hvr_batch_reserve(hvr_batch *b, int max_requests, unsigned long long max_pixels, unsigned long long max_bricks)
+{
+    if (!b || max_requests < 0) return fail(HVR_EINVAL, "bad argument");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    // growing frees buffers the previous run may still read
+    HIPCHK(hipStreamSynchronize(b->stream));
+    const size_t nreq = (size_t)max_requests;
+    // the largest staged block: header, descriptors, LUTs of 2048 entries
+    // and their prefix counts (the layout of hvr_batch_run_async)
+    const size_t desc_off = align_up(sizeof(BatchHeader), 256);
+    const size_t lut_off = align_up(desc_off + nreq * sizeof(ReqDesc), 256);
+    const size_t nz_off = align_up(lut_off + nreq * kLutLdsMax * sizeof(float4), 256);
+    const size_t stage = align_up(nz_off + nreq * (kLutLdsMax + 1) * sizeof(int), 256);
+    const size_t pixels = (size_t)max_pixels + 64 * nreq;   // each request padded to 64 pixels
+    const size_t flags = nreq * align_up((size_t)max_bricks, 256);
+    bool moved = false, moved_out = false;
+    if (const int rc = grow((void **)&b->d_stage, (void **)&b->h_stage, &b->stage_cap, stage, &moved)) return rc;
+    if (const int rc = grow((void **)&b->d_out, (void **)&b->h_out, &b->out_cap, pixels * 4, &moved_out)) return rc;
+    if (const int rc = grow((void **)&b->d_flags, nullptr, &b->flags_cap, flags, &moved_out)) return rc;
+    if (moved) batch_drop_graph(b);
     return HVR_OK;
 }
 

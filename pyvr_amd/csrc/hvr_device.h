@@ -552,6 +552,7 @@ struct FrameConst {
     uint32_t clear[4];  // framebuffer clear colour (8-bit per channel)
     int blend;          // 1: SRC_ALPHA, ONE_MINUS_SRC_ALPHA over the clear colour
     int count_shaded;   // counting instances: count only samples that load texels
+    uint8_t *touched;   // counting instances: set to 1 for every brick a sample loads from (or null)
 };
 
 // One ray per pixel, produced by the setup kernel.
@@ -846,6 +847,9 @@ __device__ __forceinline__ void march_ray(const FrameConst &fc, const TexDesc &t
         auto shade = [&](const Footprint &fb) {
             Footprint f = fb;
             footprint_addr<T, LB, BUF>(ts, f);
+            if constexpr (COUNT) {
+                if (fc.touched) fc.touched[f.brick] = 1;   // the bricks the frame reads
+            }
             if constexpr (NM == NM_SAME) {
                 constexpr int T = texel_bytes(SF);
                 uint32_t raw[4][T / 2];

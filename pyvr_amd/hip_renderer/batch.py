@@ -226,6 +226,15 @@ class BatchRenderer:
             check(self.L.hvr_batch_set_kernel(s.handle, w, int(sk), int(g)), "hvr_batch_set_kernel")
         self.kernel = kernel
 
+    def reserve(self, max_requests: int, max_pixels: int, max_bricks: int):
+        """Grow every slot's pools now to what a batch of up to
+        ``max_requests`` requests, ``max_pixels`` pixels and volumes of up to
+        ``max_bricks`` bricks needs, so that such batches never allocate
+        while others are in flight (hvr_batch_reserve)."""
+        for s in self._slots:
+            check(self.L.hvr_batch_reserve(s.handle, int(max_requests), int(max_pixels), int(max_bricks)),
+                  "hvr_batch_reserve")
+
     def set_tile_shape(self, tile_shape: str):
         """Ray tile of a wavefront for the following launches ("8x8" or "16x4")."""
         if tile_shape not in TILE_SHAPES:

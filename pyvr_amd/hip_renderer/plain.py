@@ -150,6 +150,14 @@ class PlainRenderer:
               "hvr_shaded_sample_counts")
         return c.reshape(h, w)
 
+    def touched_bytes(self) -> int:
+        """Compulsory texel bytes of the prepared request's frame: the bricks
+        its samples load texels from, times the bytes of a brick."""
+        t, n, b = ctypes.c_ulonglong(), ctypes.c_ulonglong(), ctypes.c_ulonglong()
+        check(self.L.hvr_touched_bricks(self.ctx, ctypes.byref(t), ctypes.byref(n), ctypes.byref(b)),
+              "hvr_touched_bricks")
+        return int(t.value) * int(b.value)
+
     def download(self, volume, normals: bool = True):
         """The resident volume's stored fields (scalar as the march reads it,
         normals as stored) as float32 arrays, for checking on the CPU."""

@@ -218,7 +218,14 @@ class RenderServer:
         reserved = self._reserve(max(self.max_bricks, volume_bricks(shape, self.brick_log2)))
         if nbytes > self.memory_budget - reserved:
             raise ValueError(f"volume {key!r} needs {nbytes} bytes, more than the budget allows")
-        self.max_bricks = max(self.max_bricks, volume_bricks(shape, self.brick_log2))
+        bricks = volume_bricks(shape, self.brick_log2)
+        if bricks > self.max_bricks:
+            # the batches' pools at their largest now (what the budget
+            # reserves), so that no launch grows one while others run
+            reserve = getattr(self.backend, "reserve", None)
+            if reserve is not None:
+                reserve(self.max_batch_requests, self.max_batch_pixels, bricks)
+            self.max_bricks = bricks
         self.reserved = reserved
         self._registry[key] = (volume, device_normals, nbytes)
 

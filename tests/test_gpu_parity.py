@@ -254,3 +254,38 @@ def test_shaded_sample_counts(hip):
     print(f"[shaded] samples {int(cnt1.sum())}, shaded {int(sh1.sum())} with skipping", flush=True)
     assert np.array_equal(sh0, cnt0) and np.array_equal(cnt0, cnt1) and np.array_equal(img0, img1)
     assert (sh1 <= cnt1).all() and 0 < sh1.sum() < cnt1.sum()
+
+
+def test_touched_bricks(hip):
+    """hvr_touched_bricks: the bricks a frame's samples load texels from.
+    Skipping can only remove bricks (those it steps over); a LUT that is
+    transparent everywhere leaves none with skipping, and without skipping
+    at least as many as the opaque LUT (no ray terminates early); the count
+    never exceeds the volume's bricks."""
+    import ctypes
+
+    from pyvr_amd.hip_renderer import _lib
+
+    sc = scenes.c2(size=96, image=128)
+    r = _prepared(sc)
+    L, ctx = _lib.lib(), r.gl_manager.ctx
+
+    def touched(lut, mode):
+        _render_raw(r, lut, mode)
+        t, n, b = ctypes.c_ulonglong(), ctypes.c_ulonglong(), ctypes.c_ulonglong()
+        _lib.check(L.hvr_touched_bricks(ctx, ctypes.byref(t), ctypes.byref(n), ctypes.byref(b)), "touched")
+        return t.value, n.value, b.value
+
+    try:
+        clear = sc.lut().copy()
+        clear[:, 3] = 0.0
+        on, nb, bb = touched(sc.lut(), _lib.SKIP_ON)
+        off, _, _ = touched(sc.lut(), _lib.SKIP_OFF)
+        clear_on, _, _ = touched(clear, _lib.SKIP_ON)
+        clear_off, _, _ = touched(clear, _lib.SKIP_OFF)
+    finally:
+        r.close()
+    print(f"[touched] {on} of {nb} bricks ({bb} B each) with skipping, {off} without; transparent LUT "
+          f"{clear_on} / {clear_off}", flush=True)
+    assert 0 < on <= off <= nb and bb > 0
+    assert clear_on == 0 and off <= clear_off <= nb
