@@ -516,6 +516,7 @@ def serving_leg(args, rates, duration, nvol, plain_requests):
                 "batches": srv.batches}
 
     run_rate(max(rates), min(64, n_max))      # warm-up: graphs for the shapes in use
+    srv.freeze_heap()                         # no full garbage collection inside the timed runs
     table = []
     for rate in rates:
         n = max(50, int(rate * duration))

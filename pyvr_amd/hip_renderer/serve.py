@@ -241,6 +241,21 @@ class RenderServer:
     def resident_bytes(self) -> int:
         return sum(r.bytes for r in self._resident.values())
 
+    @staticmethod
+    def freeze_heap():
+        """Collect once and move every object alive now into the garbage
+        collector's permanent generation (``gc.freeze``).  The serving loop
+        runs in the calling thread, so a full collection over a large
+        long-lived heap (volumes, requests, the application's own objects)
+        stalls launches and collections for its whole duration -- 17-35 ms
+        measured with the serving benchmark's heap, which then sets the p99.
+        Call it after setup, before taking traffic; objects created later
+        are collected as usual.  Process-wide."""
+        import gc
+
+        gc.collect()
+        gc.freeze()
+
     def resident_keys(self) -> List[Hashable]:
         return list(self._resident)
 

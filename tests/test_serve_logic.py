@@ -321,3 +321,20 @@ def test_cancel_queued_and_in_flight_requests():
     assert srv.poll() == [d] and srv.pending() == 0
     assert not srv.cancel(a)                     # already completed
     assert not srv.cancel(req("A"))              # never submitted
+
+
+def test_freeze_heap_moves_live_objects_out_of_collection():
+    """RenderServer.freeze_heap: what is alive at setup is no longer scanned
+    by the collector (gc.freeze), so a full collection during serving costs
+    only the objects created since."""
+    import gc
+
+    from pyvr_amd.hip_renderer.serve import RenderServer
+
+    keep = [[i] for i in range(1000)]
+    try:
+        RenderServer.freeze_heap()
+        assert gc.get_freeze_count() >= len(keep)
+    finally:
+        gc.unfreeze()
+    assert gc.get_freeze_count() == 0
