@@ -104,12 +104,17 @@ def estimated_samples(width: int, height: int, camera, config, bounds=None) -> f
     ext = np.maximum(hi - lo, 1e-12)
     centre = 0.5 * (lo + hi)
     radius = 0.5 * float(np.linalg.norm(ext))
-    try:
-        pos = np.asarray(camera.get_camera_vectors()[0], np.float64)
-        fov = float(getattr(camera, "fov", math.pi / 4))
-    except AttributeError:
-        pos, fov = centre + 3.0 * radius, math.pi / 4
-    d = float(np.linalg.norm(pos - centre))
+    fov = float(getattr(camera, "fov", math.pi / 4))
+    dist, target = getattr(camera, "distance", None), getattr(camera, "target", None)
+    if dist is not None and target is not None:
+        # the camera sits at `distance` from its target (the placement
+        # rotates an offset of that length), so at least
+        # distance - |target - centre| from the centre: the nearer bound
+        # gives the larger disc
+        off = float(np.linalg.norm(np.asarray(target, np.float64) - centre))
+        d = max(float(dist) - off, 0.0)
+    else:
+        d = 3.0 * radius
     if d <= radius:
         frac = 1.0
     else:
