@@ -338,3 +338,46 @@ def test_freeze_heap_moves_live_objects_out_of_collection():
     finally:
         gc.unfreeze()
     assert gc.get_freeze_count() == 0
+
+
+def test_pools_reserved_when_a_larger_volume_registers():
+    """The server grows the backend's batch pools to its limits (for the
+    largest registered volume) at registration, so no launch grows a pool
+    while others are in flight; smaller volumes reserve nothing more."""
+    class ReservingBackend(FakeBackend):
+        def __init__(self, pipeline=2):
+            super().__init__(pipeline)
+            self.reserved = []
+
+        def reserve(self, requests, pixels, bricks):
+            self.reserved.append((requests, pixels, bricks))
+
+    be = ReservingBackend(2)
+    srv = RenderServer(backend=be, memory_budget=1 << 30, pipeline=2, max_batch_requests=32,
+                       max_batch_pixels=1 << 12)
+    srv.register("A", vol("A", 32))
+    srv.register("B", vol("B", 16))
+    srv.register("C", vol("C", 64))
+    assert be.reserved == [(32, 1 << 12, volume_bricks((32,) * 3)), (32, 1 << 12, volume_bricks((64,) * 3))]
+
+
+def test_estimate_uses_the_camera_distance_bound():
+    """estimated_samples places the camera at its distance from the target,
+    less the target's offset from the box centre (the nearest it can be):
+    a target off the centre can only raise the estimate."""
+    from pyvr_amd.camera import Camera
+    from pyvr_amd.config import RenderConfig
+    from pyvr_amd.hip_renderer.serve import estimated_samples
+
+    cfg = RenderConfig.balanced()
+    centred = Camera.from_spherical(target=np.zeros(3, np.float32), azimuth=0.3, elevation=0.2, roll=0.0,
+                                    distance=6.0)
+    off = Camera.from_spherical(target=np.array([0.0, 1.0, 0.0], np.float32), azimuth=0.3, elevation=0.2,
+                                roll=0.0, distance=6.0)
+    inside = Camera.from_spherical(target=np.zeros(3, np.float32), azimuth=0.3, elevation=0.2, roll=0.0,
+                                   distance=1.0)
+    a, b = estimated_samples(128, 128, centred, cfg), estimated_samples(128, 128, off, cfg)
+    assert 0 < a < b
+    # inside the bounding sphere every pixel may hit the box
+    assert estimated_samples(128, 128, inside, cfg) == pytest.approx(128 * 128 * min(cfg.max_steps,
+                                                                                    4 * 8 / 24 / cfg.step_size))
