@@ -224,3 +224,30 @@ def test_batch_kernel_variants_equal_default(hip, kernel):
         finally:
             b.close()
     assert out[kernel] == out["default"]
+
+
+def test_batch_reserved_pools(hip):
+    """hvr_batch_reserve grows a batch's pools to given limits up front:
+    batches within them render the same bytes as without the reservation,
+    a smaller reservation afterwards changes nothing, and a reservation the
+    device cannot hold fails with an error instead of faulting."""
+    from pyvr_amd.hip_renderer import _lib
+
+    vols = scenes.c3_volumes(scale=4)
+    reqs = scenes.c3_requests(8, seed=11, image_scale=4)
+    b, idx = make_batch(vols, reqs)
+    try:
+        b.run()
+        plain = b.results()
+        bricks = max(int(np.prod([(s >> 3) + 1 for s in v.data.shape])) for v in vols)
+        pixels = sum(q.width * q.height for q in reqs)
+        b.reserve(16, 2 * pixels, bricks)
+        b.run()
+        assert b.results() == plain
+        b.reserve(1, 1, 1)
+        b.run()
+        assert b.results() == plain
+        with pytest.raises(RuntimeError):
+            b.reserve(16, 1 << 40, bricks)      # 4 TiB of output pool
+    finally:
+        b.close()
