@@ -24,14 +24,20 @@ using namespace hvr;
 // errors
 static thread_local std::string g_err;
 
+// A failed HIP call (an allocation above the device's memory, say) also
+// sets the thread's last error, which the launch checks of later calls read:
+// once a failure is reported (or handled), the last error is cleared so that
+// a later, unrelated call does not report it again.
 static int fail(int code, const char *msg)
 {
     g_err = msg;
+    (void)hipGetLastError();
     return code;
 }
 static int fail_hip(hipError_t e, const char *where)
 {
     g_err = std::string(where) + ": " + hipGetErrorString(e);
+    (void)hipGetLastError();
     return e == hipErrorOutOfMemory ? HVR_ENOMEM : HVR_EDEVICE;
 }
 #define HIPCHK(expr)                                   \
@@ -268,15 +274,21 @@ static const uint8_t *brick_flags(hvr_ctx *c, const hvr_volume *v)
 {
     if (c->skip_mode == HVR_SKIP_OFF || !v || !v->d_range || !c->d_nz) return nullptr;
     const uint64_t nb = (uint64_t)v->nbx * v->nby * v->nbz;
+    // without flags the frame takes the non-skipping kernel: a failure here
+    // is handled, not reported
+    const auto none = [] {
+        (void)hipGetLastError();
+        return (const uint8_t *)nullptr;
+    };
     if (!(c->flags_vol == v && c->flags_vol_gen == v->gen && c->flags_lut_gen == c->lut_gen)) {
         c->flags_vol = nullptr;
-        if (!c->d_nclear && hipMalloc(&c->d_nclear, sizeof(uint32_t)) != hipSuccess) return nullptr;
+        if (!c->d_nclear && hipMalloc(&c->d_nclear, sizeof(uint32_t)) != hipSuccess) return none();
         if (nb > c->flags_cap) {
-            if (hipStreamSynchronize(c->stream) != hipSuccess) return nullptr;
+            if (hipStreamSynchronize(c->stream) != hipSuccess) return none();
             (void)hipFree(c->d_flags);
             c->d_flags = nullptr;
             c->flags_cap = 0;
-            if (hipMalloc(&c->d_flags, nb) != hipSuccess) return nullptr;
+            if (hipMalloc(&c->d_flags, nb) != hipSuccess) return none();
             c->flags_cap = nb;
         }
         // the count comes back asynchronously (no wait in the frame's path)
@@ -288,7 +300,7 @@ static const uint8_t *brick_flags(hvr_ctx *c, const hvr_volume *v)
             hipMemcpyAsync(c->h_nclear, c->d_nclear, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream) !=
                 hipSuccess ||
             hipEventRecord(c->flags_ev, c->stream) != hipSuccess)
-            return nullptr;
+            return none();
         c->flags_vol = v;
         c->flags_vol_gen = v->gen;
         c->flags_lut_gen = c->lut_gen;
@@ -1716,7 +1728,10 @@ int hvr_batch_query(hvr_batch *b, int *done)
     HIPCHK(hipSetDevice(b->ctx->device));
     const hipError_t e = hipEventQuery(b->ev2);
     if (e == hipSuccess) *done = 1;
-    else if (e == hipErrorNotReady) *done = 0;
+    else if (e == hipErrorNotReady) {
+        *done = 0;
+        (void)hipGetLastError();   // still in flight, not an error
+    }
     else return fail_hip(e, "hvr_batch_query");
     return HVR_OK;
 }

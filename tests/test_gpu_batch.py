@@ -230,9 +230,9 @@ def test_batch_reserved_pools(hip):
     """hvr_batch_reserve grows a batch's pools to given limits up front:
     batches within them render the same bytes as without the reservation,
     a smaller reservation afterwards changes nothing, and a reservation the
-    device cannot hold fails with an error instead of faulting."""
-    from pyvr_amd.hip_renderer import _lib
-
+    device cannot hold fails with an error instead of faulting -- leaving no
+    error behind for later calls on the thread (a failed allocation also
+    sets HIP's last error, which the library's launch checks read)."""
     vols = scenes.c3_volumes(scale=4)
     reqs = scenes.c3_requests(8, seed=11, image_scale=4)
     b, idx = make_batch(vols, reqs)
@@ -249,5 +249,9 @@ def test_batch_reserved_pools(hip):
         assert b.results() == plain
         with pytest.raises(RuntimeError):
             b.reserve(16, 1 << 40, bricks)      # 4 TiB of output pool
+        b.run(count=True)
+        # a new renderer (volume upload, flags and march launches) and the
+        # batch itself after the failure
+        check_against_single(b, idx, vols, reqs[:2], "f16", "sn8x3")
     finally:
         b.close()
