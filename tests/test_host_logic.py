@@ -144,6 +144,87 @@ class TestRendererSurface:
         r.render()
 
 
+class TestReferenceRendererBehaviour:
+    """The reference's renderer tests that the classes above do not already
+    cover (tests/test_moderngl_renderer/test_volume_renderer.py:423-790):
+    the TypeError texts, uniforms written by the light, volume and config
+    setters, and the presets."""
+
+    @pytest.mark.parametrize("call,kind", [
+        (lambda r: r.load_volume(np.random.random((10, 10, 10))), "Volume"),
+        (lambda r: r.set_light("not a light"), "Light"),
+        (lambda r: r.set_light({"ambient": 0.5}), "Light"),
+        (lambda r: r.set_config("not a config"), "RenderConfig"),
+        (lambda r: r.set_config({"step_size": 0.01}), "RenderConfig"),
+    ])
+    def test_type_error_messages(self, fake, call, kind):
+        r = VolumeRenderer(width=16, height=16)
+        with pytest.raises(TypeError, match=f"Expected {kind} instance"):
+            call(r)
+        with pytest.raises(TypeError, match="Expected RenderConfig instance"):
+            VolumeRenderer(config="invalid")
+
+    def test_light_uniforms(self, fake):
+        r = VolumeRenderer(width=16, height=16)
+        v = r.gl_manager.program.values
+        assert v["ambient_light"] == pytest.approx(0.2) and v["diffuse_light"] == pytest.approx(0.8)
+        L = Light.directional(direction=[1, -1, 0], ambient=0.3, diffuse=0.9)
+        r.set_light(L)
+        assert r.light is L
+        assert v["ambient_light"] == pytest.approx(0.3) and v["diffuse_light"] == pytest.approx(0.9)
+        assert np.allclose(v["light_position"], L.position) and np.allclose(v["light_target"], L.target)
+        P = Light.point_light(position=[5, 5, 5])
+        r.set_light(P)
+        assert np.allclose(r.light.position, [5, 5, 5]) and np.allclose(v["light_position"], [5, 5, 5])
+        r.set_light(Light.ambient_only(intensity=0.4))
+        assert v["ambient_light"] == pytest.approx(0.4) and v["diffuse_light"] == 0.0
+        custom = Light.directional(direction=[1, 0, 0], ambient=0.5)
+        r2 = VolumeRenderer(width=16, height=16, light=custom)
+        assert r2.light is custom and r2.gl_manager.program.values["ambient_light"] == pytest.approx(0.5)
+
+    def test_volume_uniforms_and_replacement(self, fake):
+        r = VolumeRenderer(width=16, height=16)
+        assert r.get_volume() is None
+        v = r.gl_manager.program.values
+        a = Volume(data=np.random.rand(32, 32, 32).astype(np.float32), name="volume1",
+                   min_bounds=np.array([-1.0, -2.0, -3.0], np.float32),
+                   max_bounds=np.array([1.0, 2.0, 3.0], np.float32))
+        r.load_volume(a)
+        assert r.get_volume() is a and v["volume_texture"] == 0
+        assert np.allclose(v["volume_min_bounds"], (-1, -2, -3)) and np.allclose(v["volume_max_bounds"], (1, 2, 3))
+        b = Volume(data=np.random.rand(64, 64, 64).astype(np.float32), name="volume2")
+        r.load_volume(b)
+        assert r.get_volume() is b and v["volume_texture"] == 1
+        assert np.allclose(v["volume_min_bounds"], -0.5) and np.allclose(v["volume_max_bounds"], 0.5)
+        n = Volume(data=np.random.rand(32, 32, 32).astype(np.float32),
+                   normals=np.random.rand(32, 32, 32, 3).astype(np.float32))
+        r.load_volume(n)
+        assert v["volume_texture"] == 2 and v["normal_volume"] == 3
+
+    def test_config_presets_and_changes(self, fake):
+        for preset in (RenderConfig.preview(), RenderConfig.fast(), RenderConfig.balanced(),
+                       RenderConfig.high_quality(), RenderConfig.ultra_quality()):
+            r = VolumeRenderer(width=16, height=16, config=preset)
+            assert r.config is preset and r.get_config() is preset
+            v = r.gl_manager.program.values
+            assert v["step_size"] == pytest.approx(preset.step_size) and v["max_steps"] == preset.max_steps
+        r = VolumeRenderer(width=16, height=16)
+        v = r.gl_manager.program.values
+        for cfg in (RenderConfig.fast(), RenderConfig.high_quality(), RenderConfig.ultra_quality()):
+            r.set_config(cfg)
+            assert r.config is cfg
+            assert v["step_size"] == pytest.approx(cfg.step_size) and v["max_steps"] == cfg.max_steps
+            assert v["reference_step_size"] == pytest.approx(cfg.reference_step_size)
+        assert r.config.step_size == 0.001 and r.config.max_steps == 2000
+        m = RenderConfig.balanced().with_step_size(0.008)
+        r = VolumeRenderer(width=16, height=16, config=m)
+        assert r.config.step_size == 0.008 and r.config.max_steps == 500
+        r = VolumeRenderer(width=16, height=16, config=RenderConfig(step_size=0.015, max_steps=300))
+        assert r.gl_manager.program.values["max_steps"] == 300
+        r.render()
+        assert fake.params[-1]["step"] == pytest.approx(0.015) and fake.params[-1]["max_steps"] == 300
+
+
 class TestManager:
     def test_program_required(self, fake):
         m = HipManager()
