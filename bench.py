@@ -14,7 +14,7 @@ Default run (what one ``python bench.py`` prints, one JSON line):
 * batched leg (``value``): the 24 cameras as 24 requests of one persistent
   graph-captured launch per step (BatchRenderer; brick size and ray tile
   from the autotuner), two steps in flight, every step ending with its 24
-  images in pinned host memory;
+  images in pinned host memory (the march writes them there directly);
 * plain leg (``plain_kernel``): the same frames one launch (ray setup +
   march) per frame -- the single-frame kernels VolumeRenderer.render()
   runs -- back to back into resident device buffers; ``frame_ms_p50/p99``
@@ -388,7 +388,8 @@ def batched_leg(wl: Workload, args, sf, nf, s_step):
             ", ".join(f"lb{a}/{t}/{k} {ms:.2f}" for a, t, k, ms in tuning["table"]) + f" ms -> lb{lb}/{tile}/{kernel}")
     else:
         lb, tile, kernel = (args.brick_log2 if args.brick_log2 >= 0 else 3), args.tile_shape, args.kernel
-    b = BatchRenderer(scalar_format=sf, normal_format=nf, brick_log2=lb, tile_shape=tile, pipeline=2, kernel=kernel)
+    b = BatchRenderer(scalar_format=sf, normal_format=nf, brick_log2=lb, tile_shape=tile, pipeline=2, kernel=kernel,
+                      readback=args.batch_readback)
     try:
         for q in reqs:
             b.add(vols[0], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
@@ -430,7 +431,7 @@ def batched_leg(wl: Workload, args, sf, nf, s_step):
             "frames_per_s": len(reqs) * args.steps / wall, "device_ms_per_run": dev,
             "batch_ms_p50": float(np.percentile(lat, 50)), "batch_ms_p99": float(np.percentile(lat, 99)),
             "brick_log2": lb, "tile_shape": tile,
-            "kernel": kernel,
+            "kernel": kernel, "readback": args.batch_readback,
             "autotune": None if tuning is None else [{"brick_log2": a, "tile_shape": t, "kernel": k, "device_ms": ms}
                                                      for a, t, k, ms in tuning["table"]]}
 
@@ -459,7 +460,8 @@ def serving_leg(args, rates, duration, nvol, plain_requests):
     srv = RenderServer(scalar_format=sf, normal_format=nf, brick_log2=lb, tile_shape=tile,
                        max_batch_requests=args.serve_max_requests, max_batch_pixels=1 << 26,
                        pipeline=args.serve_pipeline,
-                       keep_results=False, max_batch_samples=args.serve_max_samples or None)
+                       keep_results=False, max_batch_samples=args.serve_max_samples or None,
+                       readback=args.batch_readback)
     t0 = time.perf_counter()
     for k, v in enumerate(vols):
         srv.register(k, v)
@@ -562,6 +564,7 @@ def serving_leg(args, rates, duration, nvol, plain_requests):
         f"{plain['samples_per_s'] / 1e9:.1f} G samples/s")
     best = sat if sat is not None else max(table, key=lambda r: r["samples_per_s"])
     return {"volumes": nvol, "scalar_format": sf, "normal_format": nf, "brick_log2": lb, "tile_shape": tile,
+            "readback": args.batch_readback,
             "max_batch_samples": args.serve_max_samples, "pipeline": args.serve_pipeline,
             "mean_samples_per_request": float(samples.mean()),
             "rates": table, "saturation": sat, "plain": plain,
@@ -602,7 +605,8 @@ def bench_batch(args):
     else:
         lb, tile, kernel = (args.brick_log2 if args.brick_log2 >= 0 else 3), args.tile_shape, args.kernel
 
-    b = BatchRenderer(scalar_format=sf, normal_format=nf, brick_log2=lb, tile_shape=tile, pipeline=2, kernel=kernel)
+    b = BatchRenderer(scalar_format=sf, normal_format=nf, brick_log2=lb, tile_shape=tile, pipeline=2, kernel=kernel,
+                      readback=args.batch_readback)
     for q in reqs:
         b.add(vols[q.volume], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
     b.run(count=True)
@@ -679,7 +683,7 @@ def bench_batch(args):
                                f"96x160x224, images 256^2-1024^2), one persistent graph launch per run",
                    "requests": len(reqs), "pixels_per_step": pixels, "samples_per_step": samples,
                    "scalar_format": sf, "normal_format": nf, "brick_log2": lb, "tile_shape": tile,
-                   "kernel": kernel, "autotuned": tuning is not None},
+                   "kernel": kernel, "autotuned": tuning is not None, "readback": args.batch_readback},
         "autotune": None if tuning is None else [{"brick_log2": a, "tile_shape": t, "kernel": k, "device_ms": ms}
                                                  for a, t, k, ms in tuning["table"]],
         "batch_device_ms": float(np.mean(dev_ms)), "batched_one_at_a_time": value_sync,
@@ -746,6 +750,9 @@ def main():
     ap.add_argument("--readback", choices=tuple(READBACK_MODES), default="direct",
                     help="per-frame latency leg: frames copied out of the device framebuffer, or written by the "
                          "march straight into pinned host memory")
+    ap.add_argument("--batch-readback", choices=tuple(READBACK_MODES), default="direct",
+                    help="batched and serving legs: images copied out of a device pool after each launch, or "
+                         "written by the march straight into pinned host memory")
     ap.add_argument("--latency-frames", type=int, default=48)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -776,7 +783,7 @@ def main():
     if args.workload == "c2" and not args.plain_only:
         batched = batched_leg(wl, args, sf, nf, s_step)
         bkey = (f"{wl.name}/{len(wl.cams)}cams/{sf}+{nf}/batch/lb{batched['brick_log2']}/{batched['tile_shape']}"
-                f"/{batched['kernel']}")
+                f"/{batched['kernel']}/{batched['readback']}")
         # the skipping variants step over the same samples as the single-frame
         # kernel; the noskip variant loads texels for every sample
         sh = plain["shaded_fraction"] if batched["kernel"] != "noskip" else 1.0

@@ -344,14 +344,21 @@ int hvr_batch_set_tile_shape(hvr_batch *batch, int shape);
  * Supported: (4, 1, 1) -- the default --, (2, 1, 1), (8, 1, 1), (4, 1, 0),
  * (4, 0, 1).  Every variant gives the same image; the autotuner times them. */
 int hvr_batch_set_kernel(hvr_batch *batch, int waves_per_block, int skip, int gate);
+/* Where the batch's march writes its images (no counterpart in the
+ * reference): HVR_READBACK_DIRECT (the default) straight into the pinned
+ * host pool through its device address, with no copy after the launch;
+ * HVR_READBACK_COPY into a device pool copied to the pinned pool after the
+ * launch.  Same bytes either way; applies from the next run. */
+int hvr_batch_set_readback(hvr_batch *batch, int mode);
 /* Add a request; lut_rgba holds lut_n RGBA float entries (lut_n = 0: no
  * LUT bound, reads (0, 0, 0, 1)); *index receives the request's index. */
 int hvr_batch_add(hvr_batch *batch, hvr_volume *scalar_vol, int scalar_role, hvr_volume *normal_vol,
                   int normal_role, const float *lut_rgba, int lut_n, const hvr_params *params, int width,
                   int height, int *index);
 /* Enqueue the whole batch on the batch's stream (count != 0: also record
- * per-pixel sample counts); the images are copied to pinned host memory.
- * hvr_batch_run waits for them. */
+ * per-pixel sample counts); the images end in pinned host memory (copied
+ * there, or written there, per hvr_batch_set_readback).  hvr_batch_run
+ * waits for them. */
 /* Grow the batch's pools now to what a batch of up to `max_requests`
  * requests, `max_pixels` pixels in all and volumes of up to `max_bricks`
  * bricks needs (outputs, brick flags, staged block with LUTs of up to 2048

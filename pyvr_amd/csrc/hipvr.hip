@@ -1343,8 +1343,11 @@ struct hvr_batch {
     char *h_stage, *d_stage;              // staged block (pinned host, device)
     size_t stage_cap;
     uchar4 *d_out;                        // output pool, one RGBA8 image per request
-    uint8_t *h_out;
+    uint8_t *h_out;                       // its pinned host copy
+    uchar4 *h_out_dev;                    // h_out's device address (HVR_READBACK_DIRECT)
+    const uint8_t *h_out_dev_of;          // the h_out that h_out_dev maps
     size_t out_cap;
+    int readback;                         // HVR_READBACK_DIRECT (default) / HVR_READBACK_COPY
     uint32_t *d_counts, *h_counts;        // sample-count pool (counting runs)
     size_t counts_cap;                    // in pixels
     uint8_t *d_flags;                     // transparent-brick flags, per request
@@ -1428,6 +1431,7 @@ int hvr_batch_create(hvr_ctx *c, hvr_batch **out)
     b->ctx = c;
     b->tile_shape = HVR_TILE_8X8;
     b->variant = BV_DEFAULT;
+    b->readback = HVR_READBACK_DIRECT;
     b->exec = nullptr;
     b->graph = nullptr;
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
@@ -1479,6 +1483,18 @@ int hvr_batch_set_tile_shape(hvr_batch *b, int shape)
     if (!b) return fail(HVR_EINVAL, "batch is null");
     if (shape != HVR_TILE_8X8 && shape != HVR_TILE_16X4) return fail(HVR_EINVAL, "unknown tile shape");
     b->tile_shape = shape;
+    return HVR_OK;
+}
+
+int hvr_batch_set_readback(hvr_batch *b, int mode)
+{
+    if (!b) return fail(HVR_EINVAL, "batch is null");
+    if (mode != HVR_READBACK_COPY && mode != HVR_READBACK_DIRECT)
+        return fail(HVR_EINVAL, "readback mode must be HVR_READBACK_COPY or HVR_READBACK_DIRECT");
+    // the next run's descriptors point the march at the chosen pool; a run
+    // in flight keeps the one it was enqueued with, and its images end in
+    // the pinned pool either way
+    b->readback = mode;
     return HVR_OK;
 }
 
@@ -1552,6 +1568,7 @@ int hvr_batch_reserve(hvr_batch *b, int max_requests, unsigned long long max_pix
     if (const int rc = grow((void **)&b->d_stage, (void **)&b->h_stage, &b->stage_cap, stage, &moved)) return rc;
     if (const int rc = grow((void **)&b->d_out, (void **)&b->h_out, &b->out_cap, pixels * 4, &moved_out)) return rc;
     if (const int rc = grow((void **)&b->d_flags, nullptr, &b->flags_cap, flags, &moved_out)) return rc;
+    if (moved_out) b->h_out_dev_of = nullptr;   // a new pinned pool may reuse the old address
     if (moved) batch_drop_graph(b);
     return HVR_OK;
 }
@@ -1626,7 +1643,17 @@ int hvr_batch_run_async(hvr_batch *b, int count)
         if (const int rc = grow((void **)&b->d_counts, (void **)&b->h_counts, &cb, pixels * 4, &moved_out)) return rc;
         b->counts_cap = cb / 4;
     }
+    if (moved_out) b->h_out_dev_of = nullptr;   // a new pinned pool may reuse the old address
     if (moved) batch_drop_graph(b);
+    // direct: the march stores each pixel straight into the pinned pool
+    // through its device address (posted writes over the host link while
+    // the batch runs, no copy after it)
+    const bool direct = b->readback == HVR_READBACK_DIRECT;
+    if (direct && b->h_out_dev_of != b->h_out) {
+        HIPCHK(hipHostGetDevicePointer((void **)&b->h_out_dev, b->h_out, 0));
+        b->h_out_dev_of = b->h_out;
+    }
+    uchar4 *const out_pool = direct ? b->h_out_dev : b->d_out;
     // stage: header, descriptors (class by class), LUTs, prefix counts
     memset(b->h_stage, 0, desc_off);
     BatchHeader *hdr = (BatchHeader *)b->h_stage;
@@ -1658,7 +1685,7 @@ int hvr_batch_run_async(hvr_batch *b, int count)
             d.nbricks = r.vol->nbx * r.vol->nby * r.vol->nbz;
             d.flags = b->d_flags + flag_at;
             flag_at += align_up(d.nbricks, 256);
-            d.out = b->d_out + b->out_off[i];
+            d.out = out_pool + b->out_off[i];
             d.counts = count ? b->d_counts + b->out_off[i] : nullptr;
             d.ntx = (r.W + bx - 1) / bx;
             tiles += d.ntx * ((r.H + by - 1) / by);
@@ -1699,7 +1726,7 @@ int hvr_batch_run_async(hvr_batch *b, int count)
     HIPCHK(hipEventRecord(b->ev0, b->stream));
     HIPCHK(hipGraphLaunch(b->exec, b->stream));
     HIPCHK(hipEventRecord(b->ev1, b->stream));
-    HIPCHK(hipMemcpyAsync(b->h_out, b->d_out, pixels * 4, hipMemcpyDeviceToHost, b->stream));
+    if (!direct) HIPCHK(hipMemcpyAsync(b->h_out, b->d_out, pixels * 4, hipMemcpyDeviceToHost, b->stream));
     if (count) HIPCHK(hipMemcpyAsync(b->h_counts, b->d_counts, pixels * 4, hipMemcpyDeviceToHost, b->stream));
     HIPCHK(hipEventRecord(b->ev2, b->stream));
     b->out_pixels = pixels;

@@ -44,7 +44,7 @@ EXPORTS = (
     "hvr_batch_query", "hvr_device_memory", "hvr_read_pinned", "hvr_volume_download",
     "hvr_resize", "hvr_debug_inject_fault", "hvr_render_async_to_noalloc",
     "hvr_batch_set_kernel", "hvr_set_dispatch_order", "hvr_dispatch_order_info", "hvr_set_readback",
-    "hvr_shaded_sample_counts", "hvr_touched_bricks", "hvr_batch_reserve",
+    "hvr_shaded_sample_counts", "hvr_touched_bricks", "hvr_batch_reserve", "hvr_batch_set_readback",
 )
 
 # batch ray-tile shapes (hvr_batch_set_tile_shape)
@@ -128,6 +128,7 @@ def _declare(L):
         "hvr_batch_clear": ([vp], i),
         "hvr_batch_set_tile_shape": ([vp, i], i),
         "hvr_batch_set_kernel": ([vp, i, i, i], i),
+        "hvr_batch_set_readback": ([vp, i], i),
         "hvr_batch_add": ([vp, vp, i, vp, i, fp, i, vp, i, i, ctypes.POINTER(i)], i),
         "hvr_batch_run_async": ([vp, i], i),
         "hvr_batch_reserve": ([vp, i, ctypes.c_ulonglong, ctypes.c_ulonglong], i),

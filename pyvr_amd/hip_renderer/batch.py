@@ -152,16 +152,22 @@ KERNEL_VARIANTS = {"default": (4, True, True), "waves2": (2, True, True), "waves
                    "nogate": (4, True, False), "noskip": (4, False, True)}
 
 
+# Where a batch's images go (hvr_batch_set_readback): a device pool copied
+# to pinned host memory after the launch, or the pinned pool directly.
+READBACK = {"copy": _lib.READBACK_COPY, "direct": _lib.READBACK_DIRECT}
+
+
 class _Slot:
     """One ``hvr_batch`` (own stream and buffers) and its requests' sizes."""
 
-    def __init__(self, L, ctx, tile_shape, kernel="default"):
+    def __init__(self, L, ctx, tile_shape, kernel="default", readback="direct"):
         self.L = L
         self.handle = ctypes.c_void_p()
         check(L.hvr_batch_create(ctx, ctypes.byref(self.handle)), "hvr_batch_create")
         check(L.hvr_batch_set_tile_shape(self.handle, TILE_SHAPES[tile_shape]), "hvr_batch_set_tile_shape")
         w, sk, g = KERNEL_VARIANTS[kernel]
         check(L.hvr_batch_set_kernel(self.handle, w, int(sk), int(g)), "hvr_batch_set_kernel")
+        check(L.hvr_batch_set_readback(self.handle, READBACK[readback]), "hvr_batch_set_readback")
         self.sizes: List[tuple] = []
         self.in_flight = False
 
@@ -181,6 +187,11 @@ class BatchRenderer:
     wavefront ("8x8" or "16x4") and ``kernel`` the kernel variant (a key of
     ``KERNEL_VARIANTS``: waves per block, skipping, gating; every variant
     gives the same images) -- the choices the autotuner makes.
+    ``readback`` ("direct" or "copy", a key of ``READBACK``): whether the
+    march writes the images straight to pinned host memory (no copy after
+    the launch, and no copy engine reading HBM while the next batch
+    marches), or to a device pool copied to host memory after the launch
+    (same bytes).
 
     ``pipeline`` batches can be in flight at once (each has its own stream
     and buffers): ``submit()`` launches the batch being filled and moves on
@@ -189,19 +200,23 @@ class BatchRenderer:
     """
 
     def __init__(self, device: int = 0, scalar_format: str = "f16", normal_format: str = "sn8x3",
-                 brick_log2: int = 3, tile_shape: str = "8x8", pipeline: int = 2, kernel: str = "default"):
+                 brick_log2: int = 3, tile_shape: str = "8x8", pipeline: int = 2, kernel: str = "default",
+                 readback: str = "direct"):
         if tile_shape not in TILE_SHAPES:
             raise ValueError(f"tile_shape must be one of {sorted(TILE_SHAPES)}")
         if kernel not in KERNEL_VARIANTS:
             raise ValueError(f"kernel must be one of {sorted(KERNEL_VARIANTS)}")
         if pipeline < 1:
             raise ValueError("pipeline must be at least 1")
+        if readback not in READBACK:
+            raise ValueError(f"readback must be one of {sorted(READBACK)}")
         self.L = _lib.lib()
         self.scalar_format = scalar_format
         self.normal_format = normal_format
         self.brick_log2 = brick_log2
         self.tile_shape = tile_shape
         self.kernel = kernel
+        self.readback = readback
         self.ctx = ctypes.c_void_p()
         check(self.L.hvr_create(device, 1, 1, ctypes.byref(self.ctx)), "hvr_create")
         self._resident: Dict[int, ResidentVolume] = {}
@@ -209,7 +224,7 @@ class BatchRenderer:
         try:
             check(self.L.hvr_set_brick_log2(self.ctx, brick_log2), "hvr_set_brick_log2")
             for _ in range(pipeline):
-                self._slots.append(_Slot(self.L, self.ctx, tile_shape, kernel))
+                self._slots.append(_Slot(self.L, self.ctx, tile_shape, kernel, readback))
         except Exception:
             self.close()
             raise
@@ -225,6 +240,15 @@ class BatchRenderer:
         for s in self._slots:
             check(self.L.hvr_batch_set_kernel(s.handle, w, int(sk), int(g)), "hvr_batch_set_kernel")
         self.kernel = kernel
+
+    def set_readback(self, readback: str):
+        """Where the following launches write their images ("copy" or
+        "direct"; same bytes)."""
+        if readback not in READBACK:
+            raise ValueError(f"readback must be one of {sorted(READBACK)}")
+        for s in self._slots:
+            check(self.L.hvr_batch_set_readback(s.handle, READBACK[readback]), "hvr_batch_set_readback")
+        self.readback = readback
 
     def reserve(self, max_requests: int, max_pixels: int, max_bricks: int):
         """Grow every slot's pools now to what a batch of up to

@@ -184,11 +184,11 @@ class RenderServer:
                  brick_log2: int = 3, tile_shape: str = "8x8", memory_budget: Optional[int] = None,
                  max_batch_requests: int = 256, max_batch_pixels: int = 1 << 25, pipeline: int = 3,
                  keep_results: bool = True, backend=None, max_batch_samples: Optional[float] = None,
-                 kernel: str = "default"):
+                 kernel: str = "default", readback: str = "direct"):
         self.scalar_format, self.normal_format, self.brick_log2 = scalar_format, normal_format, brick_log2
         self.backend = backend if backend is not None else BatchRenderer(
             device=device, scalar_format=scalar_format, normal_format=normal_format, brick_log2=brick_log2,
-            tile_shape=tile_shape, pipeline=pipeline, kernel=kernel)
+            tile_shape=tile_shape, pipeline=pipeline, kernel=kernel, readback=readback)
         if memory_budget is None:
             free, total = _lib.device_memory(device)
             memory_budget = int(0.9 * free)

@@ -255,3 +255,47 @@ def test_batch_reserved_pools(hip):
         check_against_single(b, idx, vols, reqs[:2], "f16", "sn8x3")
     finally:
         b.close()
+
+
+def test_batch_direct_output_equals_copy(hip):
+    """hvr_batch_set_readback: the march writing straight into the pinned
+    pool (the default) gives the bytes of the copied-out device pool -- for
+    a batch, after its pools grew (new pinned pool, new device address),
+    after switching modes, and on two slots in flight."""
+    from pyvr_amd.hip_renderer import BatchRenderer
+
+    vols = scenes.c3_volumes(scale=4)
+    small = scenes.c3_requests(5, seed=41, image_scale=4)
+    large = scenes.c3_requests(12, seed=42, image_scale=2)
+    want = {}
+    for name, reqs in (("small", small), ("large", large)):
+        b, _ = make_batch(vols, reqs, pipeline=1, readback="copy")
+        try:
+            b.run(count=True)
+            want[name] = (b.results(), [b.counts(i).copy() for i in range(len(reqs))])
+        finally:
+            b.close()
+    b = BatchRenderer(pipeline=2, readback="direct")
+    try:
+        for name, reqs in (("small", small), ("large", large), ("small", small)):
+            b.clear()
+            for q in reqs:
+                b.add(vols[q.volume], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
+            b.run(count=True)
+            assert b.results() == want[name][0], name
+            assert all(np.array_equal(b.counts(i), c) for i, c in enumerate(want[name][1])), name
+        b.set_readback("copy")
+        b.run()
+        assert b.results() == want["small"][0]
+        b.set_readback("direct")
+        b.clear()
+        for q in small:
+            b.add(vols[q.volume], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
+        sa = b.submit()
+        for q in large:
+            b.add(vols[q.volume], q.camera, q.ctf, q.otf, q.width, q.height, q.config, q.light, q.tau)
+        sb = b.submit()
+        assert b.results(sa) == want["small"][0]
+        assert b.results(sb) == want["large"][0]
+    finally:
+        b.close()
