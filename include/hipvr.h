@@ -128,7 +128,8 @@ int hvr_destroy(hvr_ctx *ctx);
  * parameters' matrices are the caller's (aspect = width / height); buffers
  * grow when the new size needs more pixels and are kept otherwise; only
  * growing waits for the context's stream (frames in flight keep their own
- * sizes). */
+ * sizes).  Until the next frame at the new size, the framebuffer (and
+ * hvr_read) holds the earlier frames' pixels at their own size. */
 int hvr_resize(hvr_ctx *ctx, int width, int height);
 
 /* ---- volumes: replace create_volume_texture (manager.py:77-108) and

@@ -9,6 +9,7 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <atomic>
 #include <condition_variable>
 #include <mutex>
 #include <thread>
@@ -19,8 +20,9 @@ namespace hvr {
 // core the copy is bound by that core's memory bandwidth), split between the
 // calling thread and three worker threads.  The workers start on first use
 // (detached; they wait on a condition variable between copies) and serve one
-// copy at a time.  In a process forked after they started, copies stay on
-// the calling thread.
+// copy at a time.  In a process forked after they started (whose copies of
+// the locks may be held by threads that do not exist there), and if the
+// workers cannot be started, copies stay on the calling thread.
 class CopyPool {
 public:
     static CopyPool &get()
@@ -30,7 +32,8 @@ public:
     }
     void copy(uint8_t *dst, const uint8_t *src, size_t n)
     {
-        if (n < ((size_t)1 << 20)) {
+        const pid_t owner = owner_.load(std::memory_order_acquire);
+        if (n < ((size_t)1 << 20) || (owner != 0 && owner != getpid())) {
             memcpy(dst, src, n);
             return;
         }
@@ -62,14 +65,20 @@ private:
         const uint8_t *src;
         size_t n;
     };
-    // with mu_ held: start the workers once; false in a forked child
+    // with mu_ held: start the workers once; false when they could not all
+    // be started (those that did wait for a generation that never comes)
     bool start()
     {
-        if (pid_ == 0) {
-            pid_ = getpid();
-            for (int w = 0; w < kWorkers; ++w) std::thread([this, w] { run(w); }).detach();
+        if (owner_.load(std::memory_order_relaxed) == 0) {
+            int started = 0;
+            try {
+                for (; started < kWorkers; ++started) std::thread([this, w = started] { run(w); }).detach();
+            } catch (...) {
+            }
+            ok_ = started == kWorkers;
+            owner_.store(getpid(), std::memory_order_release);
         }
-        return pid_ == getpid();
+        return ok_;
     }
     void run(int w)
     {
@@ -92,7 +101,8 @@ private:
     Job job_[kWorkers] = {};
     int left_ = 0;
     uint64_t gen_ = 0;
-    pid_t pid_ = 0;
+    bool ok_ = false;
+    std::atomic<pid_t> owner_{0};   // the process that started the workers
 };
 
 }  // namespace hvr

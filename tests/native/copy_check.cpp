@@ -2,10 +2,13 @@
 // ThreadSanitizer by tests/test_sanitizers.py: copies of sizes from 0 to
 // 9 MiB (below and above the 1 MiB split, not multiples of the part size)
 // at unaligned offsets equal memcpy's, from one caller and from four
-// callers at once.  Exit status 0 on success.
+// callers at once, and in a child forked after the workers started (where
+// copies stay on the calling thread).  Exit status 0 on success.
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/wait.h>
+#include <unistd.h>
 
 #include <thread>
 #include <vector>
@@ -52,6 +55,13 @@ int main()
         });
     for (auto &c : callers) c.join();
     for (int t = 0; t < 4; ++t) fails += bad[t];
+    const pid_t child = fork();
+    if (child == 0) _exit(one((size_t)3 << 20, 5, 7, 99) && one((size_t)1 << 20, 0, 0, 98) ? 0 : 1);
+    int status = 0;
+    if (child < 0 || waitpid(child, &status, 0) != child || !WIFEXITED(status) || WEXITSTATUS(status) != 0) {
+        fprintf(stderr, "FAIL forked child\n");
+        ++fails;
+    }
     if (fails) return 1;
     printf("copy ok\n");
     return 0;
