@@ -90,7 +90,6 @@ struct hvr_ctx {
     int W, H;
     hipStream_t stream;
     uchar4 *d_out;
-    RayRec *d_rays;
     uint32_t *d_counts;
     float4 *d_lut;
     int lut_n;       // entries in d_lut (>= 1)
@@ -130,9 +129,9 @@ struct hvr_ctx {
     int skip_used;         // whether the last frame's march stepped over bricks
     bool force_global;     // HVR_GATHER=global at creation: 64-bit loads for every volume
     int ncu;               // compute units of the device
-    size_t pix_cap;        // pixels d_out, d_rays and h_pinned hold
+    size_t pix_cap;        // pixels d_out and h_pinned hold
     size_t counts_cap;     // pixels d_counts holds
-    // cost-ordered dispatch of single frames (k_ray_setup_ordered)
+    // cost-ordered dispatch of single frames (k_order_blocks)
     uint32_t *d_order;     // two sets of 8 x kOrderBuckets counters, then the slots
     size_t order_words;    // words d_order holds
     int order_mode;        // HVR_ORDER_OFF / _ON / _AUTO
@@ -244,7 +243,7 @@ static void launch_march_generic(const MarchArgs &a)
 {
     const size_t lds = NM == NM_ANY ? 0 : 2 * (size_t)a.lut_n * sizeof(float4);
     hipLaunchKernelGGL((k_march<SF, NM, 3, COUNT, false, false>), dim3(a.nblocks), dim3(256), lds, a.stream, a.fc,
-                       a.ts, a.tn, a.lut, a.lut_n, a.rays, a.out, a.counts, a.skip, a.ntx, a.nblocks,
+                       a.ts, a.tn, a.lut, a.lut_n, a.out, a.counts, a.skip, a.ntx, a.nblocks,
                        a.order_cnt, a.order_slots, a.order_cap);
 }
 
@@ -332,7 +331,6 @@ static void launch_march(hvr_ctx *c, const FrameConst &fc, uchar4 *out, bool cou
     a.tn = tex_desc(N, c->normal_role, fc);
     a.lut = (const float4 *)c->d_lut;
     a.lut_n = c->lut_n;
-    a.rays = (const RayRec *)c->d_rays;
     a.out = out;
     a.counts = c->d_counts;
     a.skip = nullptr;
@@ -383,7 +381,6 @@ static int enqueue_frame(hvr_ctx *c, bool count, uchar4 *dst, bool count_shaded 
     FrameConst fc = frame_const(c);
     fc.count_shaded = count_shaded;
     fc.touched = touched;
-    const int npix = c->W * c->H;
     // events of this frame: the context's own, or the next slots of an
     // open timing region; hvr_last_kernel_ms reads the frame's events
     hipEvent_t e0 = c->ev[0], e1 = c->ev[1], e2 = c->ev[2];
@@ -394,8 +391,8 @@ static int enqueue_frame(hvr_ctx *c, bool count, uchar4 *dst, bool count_shaded 
         e2 = (*c->tev)[k + 2];
     }
     // cost-ordered dispatch (hvr_set_dispatch_order): the blocks of the
-    // first multiple-of-8 block rows are filed by ray setup and marched in
-    // decreasing order of their rays' iteration bounds
+    // first multiple-of-8 block rows are filed by k_order_blocks and marched
+    // in decreasing order of their rays' iteration bounds
     const int ntx = (c->W + 15) / 16, nty = (c->H + 15) / 16, nblocks = ntx * nty;
     const int head_rows = nty >> 3 << 3, cap = (head_rows >> 3) * ntx;
     const bool ordered = head_rows > 0 && (c->order_mode == HVR_ORDER_ON ||
@@ -427,12 +424,12 @@ static int enqueue_frame(hvr_ctx *c, bool count, uchar4 *dst, bool count_shaded 
         c->order_parity = p ^ 1;
         cnt = c->d_order + p * kSet;
         slots = c->d_order + 2 * kSet;
-        hipLaunchKernelGGL(k_ray_setup_ordered, dim3(nblocks), dim3(256), 0, c->stream, fc, c->d_rays, cnt,
+        // the blocks of the head rows only (the others keep row order)
+        hipLaunchKernelGGL(k_order_blocks, dim3(head_rows * ntx), dim3(256), 0, c->stream, fc, cnt,
                            c->d_order + (p ^ 1) * kSet, slots, ntx, head_rows, cap);
-    } else {
-        hipLaunchKernelGGL(k_ray_setup, dim3((npix + 255) / 256), dim3(256), 0, c->stream, fc, c->d_rays);
+        HIPCHK(hipGetLastError());
     }
-    HIPCHK(hipGetLastError());
+    // the march sets up its rays itself
     HIPCHK(hipEventRecord(e1, c->stream));
     const bool host = !dst && c->readback_mode == HVR_READBACK_DIRECT;
     launch_march(c, fc, dst ? dst : host ? (uchar4 *)c->h_pinned_dev : c->d_out, count, cnt, slots, cap);
@@ -554,7 +551,6 @@ int hvr_create(int device, int width, int height, hvr_ctx **out)
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device);
     if (e == hipSuccess) e = hipMalloc(&c->d_out, npix * sizeof(uchar4));
-    if (e == hipSuccess) e = hipMalloc(&c->d_rays, npix * sizeof(RayRec));
     if (e == hipSuccess) e = hipHostMalloc(&c->h_pinned, npix * 4, hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostGetDevicePointer(&c->h_pinned_dev, c->h_pinned, 0);
     if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_nclear, sizeof(uint32_t), hipHostMallocDefault);
@@ -593,16 +589,13 @@ int hvr_resize(hvr_ctx *c, int width, int height)
     if (npix > c->pix_cap) {
         HIPCHK(hipStreamSynchronize(c->stream));
         (void)hipFree(c->d_out);
-        (void)hipFree(c->d_rays);
         if (c->h_pinned) (void)hipHostFree(c->h_pinned);
         c->d_out = nullptr;
-        c->d_rays = nullptr;
         c->h_pinned = nullptr;
         c->h_pinned_dev = nullptr;
         c->last_host = false;
         c->pix_cap = 0;
         HIPCHK(hipMalloc(&c->d_out, npix * sizeof(uchar4)));
-        HIPCHK(hipMalloc(&c->d_rays, npix * sizeof(RayRec)));
         HIPCHK(hipHostMalloc(&c->h_pinned, npix * 4, hipHostMallocDefault));
         HIPCHK(hipHostGetDevicePointer(&c->h_pinned_dev, c->h_pinned, 0));
         HIPCHK(hipMemset(c->d_out, 0, npix * sizeof(uchar4)));
@@ -619,7 +612,6 @@ int hvr_destroy(hvr_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     (void)hipFree(c->d_out);
-    (void)hipFree(c->d_rays);
     (void)hipFree(c->d_counts);
     (void)hipFree(c->d_lut);
     (void)hipFree(c->d_stage);

@@ -563,8 +563,8 @@ struct __align__(16) RayRec {
     int pad;
 };
 
-// Ray setup (volume.vert.glsl:8; volume.frag.glsl:47-68, 71-82); K1 runs it
-// per pixel into a buffer, the batch kernel inline.
+// Ray setup (volume.vert.glsl:8; volume.frag.glsl:47-68, 71-82), run inline
+// by the march kernels (once per ray) and by the block-ordering pass.
 // Pixel (px, py), py = 0 the bottom row.  Positions use separately rounded
 // products and sums (no fused multiply-add), as the shader states them.
 __device__ __forceinline__ RayRec ray_setup(const FrameConst &fc, int px, int py)
@@ -617,16 +617,10 @@ __device__ __forceinline__ RayRec ray_setup(const FrameConst &fc, int px, int py
     return r;
 }
 
-static __global__ void __launch_bounds__(256) k_ray_setup(const FrameConst fc, RayRec *__restrict__ rays)
-{
-    const int idx = blockIdx.x * 256 + threadIdx.x;
-    if (idx >= fc.W * fc.H) return;
-    rays[idx] = ray_setup(fc, idx % fc.W, idx / fc.W);
-}
-
-// K1 for a cost-ordered frame: one block per 16x16 ray block (row-major,
-// ntx per row), which also files the block for the march's dispatch order
-// (ordered_block): its key is the largest iteration bound of its rays.
+// K1 -- the dispatch order of a cost-ordered frame: one block per 16x16 ray
+// block (row-major, ntx per row) sets up its rays (ray_setup, nothing
+// stored) and files the block for the march (ordered_block) under the
+// largest iteration bound of its rays.
 // `cnt` holds 8 x kOrderBuckets counters (zero on entry), `slots` cap
 // entries per class and bucket; only the blocks of the first `head_rows`
 // block rows (a multiple of 8) are filed.  Block 0 also clears `clear`, the
@@ -634,20 +628,16 @@ static __global__ void __launch_bounds__(256) k_ray_setup(const FrameConst fc, R
 // the next ordered frame fills (counters alternate between frames, so no
 // separate clearing launch is needed).
 static __global__ void __launch_bounds__(256)
-k_ray_setup_ordered(const FrameConst fc, RayRec *__restrict__ rays, uint32_t *__restrict__ cnt,
-                    uint32_t *__restrict__ clear, uint32_t *__restrict__ slots, int ntx, int head_rows, int cap)
+k_order_blocks(const FrameConst fc, uint32_t *__restrict__ cnt, uint32_t *__restrict__ clear,
+               uint32_t *__restrict__ slots, int ntx, int head_rows, int cap)
 {
     __shared__ int s_key[4];
     const int t = blockIdx.x, tx = t % ntx, ty = t / ntx;
     if (t == 0 && threadIdx.x < 8 * kOrderBuckets) clear[threadIdx.x] = 0u;
+    if (ty >= head_rows) return;   // the whole block
     const int px = tx * 16 + (threadIdx.x & 15), py = ty * 16 + (threadIdx.x >> 4);
     int key = 0;
-    if (px < fc.W && py < fc.H) {
-        const RayRec r = ray_setup(fc, px, py);
-        rays[py * fc.W + px] = r;
-        key = r.nlim;
-    }
-    if (ty >= head_rows) return;   // the whole block
+    if (px < fc.W && py < fc.H) key = ray_setup(fc, px, py).nlim;
     for (int off = 32; off > 0; off >>= 1) key = max(key, __shfl_xor(key, off, 64));
     if ((threadIdx.x & 63) == 0) s_key[threadIdx.x >> 6] = key;
     __syncthreads();
@@ -763,7 +753,7 @@ __device__ __forceinline__ void composite(const FrameConst &fc, const float4 &q,
 // contiguous bands per XCD left some XCDs idle for half the frame, while
 // interleaved rows give every XCD an even share, and its L2 still sees whole
 // rows.  Rows past the last multiple of 8 keep the plain order.  With
-// `order` (a cost-ordered frame, k_ray_setup_ordered), each XCD's class of
+// `order` (a cost-ordered frame, k_order_blocks), each XCD's class of
 // rows is taken in decreasing order of its blocks' iteration bounds
 // instead (ordered_block), so that the long blocks start in the first round
 // of resident blocks rather than after it.  A wave runs
@@ -964,7 +954,7 @@ __device__ __forceinline__ void load_lut_lds(const float4 *__restrict__ lut_g, i
 template <int SF, int NM, int LB, bool COUNT, bool SKIP, bool BUF>
 __global__ void __launch_bounds__(256)
 k_march(const FrameConst fc, const TexDesc ts, const TexDesc tn, const float4 *__restrict__ lut_g,
-        int lut_n, const RayRec *__restrict__ rays, uchar4 *__restrict__ out,
+        int lut_n, uchar4 *__restrict__ out,
         uint32_t *__restrict__ counts, const uint8_t *__restrict__ skip, int ntx, int nblocks,
         const uint32_t *__restrict__ order_cnt, const uint32_t *__restrict__ order_slots, int order_cap)
 {
@@ -977,7 +967,7 @@ k_march(const FrameConst fc, const TexDesc ts, const TexDesc tn, const float4 *_
     const int head = ((nblocks / ntx) >> 3 << 3) * ntx;   // blocks of the first multiple-of-8 rows
     int logical = b;
     if (order_cnt) {
-        // cost-ordered (k_ray_setup_ordered filed the blocks)
+        // cost-ordered (k_order_blocks filed the blocks)
         logical = ordered_block(b, head, order_cnt, order_slots, order_cap);
         if (logical < 0 || logical >= nblocks) logical = b;
     } else if (b < head) {
@@ -990,7 +980,7 @@ k_march(const FrameConst fc, const TexDesc ts, const TexDesc tn, const float4 *_
     const int py = ty * 16 + (wv >> 1) * 8 + (lane >> 3);
     if (px >= fc.W || py >= fc.H) return;
     const int pix = py * fc.W + px;
-    const RayRec r = rays[pix];
+    const RayRec r = ray_setup(fc, px, py);
 
     float C0 = 0.f, C1 = 0.f, C2 = 0.f, A = 0.f;
     uint32_t ns = 0;

@@ -15,12 +15,11 @@ struct MarchArgs {
     TexDesc ts, tn;
     const float4 *lut;
     int lut_n;
-    const RayRec *rays;
     uchar4 *out;
     uint32_t *counts;
     const uint8_t *skip;
     int ntx, nblocks;
-    const uint32_t *order_cnt;     // cost-ordered dispatch (k_ray_setup_ordered), or nullptr
+    const uint32_t *order_cnt;     // cost-ordered dispatch (k_order_blocks), or nullptr
     const uint32_t *order_slots;
     int order_cap;
     hipStream_t stream;

@@ -23,7 +23,7 @@ void march(const MarchArgs &a)
 {
     const size_t lds = 2 * (size_t)a.lut_n * sizeof(float4);
     hipLaunchKernelGGL((k_march<F, NM, LB, COUNT, SKIP, BUF>), dim3(a.nblocks), dim3(256), lds, a.stream, a.fc, a.ts,
-                       a.tn, a.lut, a.lut_n, a.rays, a.out, a.counts, a.skip, a.ntx, a.nblocks,
+                       a.tn, a.lut, a.lut_n, a.out, a.counts, a.skip, a.ntx, a.nblocks,
                        a.order_cnt, a.order_slots, a.order_cap);
 }
 
