@@ -729,8 +729,11 @@ def main():
     ap.add_argument("--size", type=int, default=0)
     ap.add_argument("--image", type=int, default=0)
     ap.add_argument("--cameras", type=int, default=24)
-    ap.add_argument("--scalar-format", default="f16", choices=["f32", "f16", "u8"])
-    ap.add_argument("--normal-format", default="f16x3", choices=["f32x3", "sn8x3", "f16x3"])
+    ap.add_argument("--scalar-format", default=None, choices=["f32", "f16", "u8"],
+                    help="volume storage (default: f32 for c0 and c1, whose configurations store float32 "
+                         "volumes; f16 otherwise)")
+    ap.add_argument("--normal-format", default=None, choices=["f32x3", "sn8x3", "f16x3"],
+                    help="normal storage (default: f32x3 for c0 and c1, f16x3 otherwise)")
     ap.add_argument("--serve-scalar-format", default="f16", choices=["f32", "f16", "u8"])
     ap.add_argument("--serve-normal-format", default="sn8x3", choices=["f32x3", "sn8x3", "f16x3"])
     ap.add_argument("--serve-rates", default="500,1000", help="serving leg of the default run (req/s)")
@@ -762,6 +765,9 @@ def main():
                     help="before the timed region, copy this many MiB device-to-device once "
                          "(a known-size kernel for calibrating traffic counters)")
     args = ap.parse_args()
+    fp32 = args.workload in ("c0", "c1")
+    args.scalar_format = args.scalar_format or ("f32" if fp32 else "f16")
+    args.normal_format = args.normal_format or ("f32x3" if fp32 else "f16x3")
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); this renderer's "
