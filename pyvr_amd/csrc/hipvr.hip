@@ -1702,9 +1702,11 @@ int hvr_batch_run_async(hvr_batch *b, int count)
             hipLaunchKernelGGL(k_batch_flags, dim3(1024), dim3(256), 0, b->stream, (const char *)b->d_stage);
             if (hipGetLastError() != hipSuccess) rc = HVR_EDEVICE;
         }
-        for (size_t k = 0; k < cls.size() && rc == HVR_OK; ++k)
-            rc = batch_fn(cls[k].fmt, cls[k].nm, cls[k].lb, count != 0, b->tile_shape, cls[k].buf, variant)(
-                b->d_stage, (int)k, b->d_counters + k, lut_cap, b->ncu, b->stream);
+        for (size_t k = 0; k < cls.size() && rc == HVR_OK; ++k) {
+            const BatchLaunchFn fn =
+                batch_fn(cls[k].fmt, cls[k].nm, cls[k].lb, count != 0, b->tile_shape, cls[k].buf, variant);
+            rc = fn ? fn(b->d_stage, (int)k, b->d_counters + k, lut_cap, b->ncu, b->stream) : HVR_EINVAL;
+        }
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(b->stream, &g);
         if (rc != HVR_OK || e != hipSuccess) {

@@ -23,7 +23,11 @@ turns a stream of requests into batches for ``BatchRenderer``:
 * a request's latency runs from its arrival to its pixels being in host
   memory; ``poll()`` returns the requests that completed, with ``rgba``
   filled (the same bytes ``VolumeRenderer.render()`` gives for it) unless
-  ``keep_results`` is off (load measurements that do not consume pixels).
+  ``keep_results`` is off (load measurements that do not consume pixels);
+* a request the batched path rejects (``BatchRenderer.add`` raises: a LUT
+  of more than 2048 entries, a volume layout it has no kernel for) or
+  whose batch fails to launch completes with ``error`` set and no pixels,
+  and the server goes on with the others.
 """
 
 import time
@@ -146,6 +150,7 @@ class ServeRequest:
     tag: object = None
     estimate: Optional[float] = None  # estimated samples (set by the server when it bounds batches by work)
     cancelled: bool = False           # set by RenderServer.cancel
+    error: Optional[str] = None       # why the request was not rendered (then rgba is None)
 
     @property
     def latency(self) -> Optional[float]:
@@ -211,6 +216,8 @@ class RenderServer:
         self.batches = 0
         self.evictions = 0
         self.cancelled = 0
+        self.failed = 0
+        self._failed: List[ServeRequest] = []   # completed with an error, returned by the next poll
 
     # -- volumes -------------------------------------------------------------
     def register(self, key: Hashable, volume, device_normals: bool = False):
@@ -324,7 +331,7 @@ class RenderServer:
     def poll(self) -> List[ServeRequest]:
         """Collect finished batches, then launch queued requests into free
         slots; never waits for the GPU.  Returns the completed requests."""
-        out = []
+        out, self._failed = self._failed, []
         for k in list(self._flight):
             if self.backend.done(k):
                 out.extend(self._collect(k))
@@ -368,34 +375,65 @@ class RenderServer:
         with trace.span("hvr.serve.admit"):
             return self._admit_and_launch()
 
+    def _fail(self, r: ServeRequest, err: BaseException):
+        r.error = f"{type(err).__name__}: {err}"
+        r.done = time.perf_counter()
+        self.failed += 1
+        self._failed.append(r)
+
     def _admit_and_launch(self) -> bool:
-        chosen, pixels, work, waiting = [], 0, 0.0, deque()
-        while self._queue and len(chosen) < self.max_batch_requests:
-            r = self._queue[0]
-            px = r.width * r.height
-            if chosen and pixels + px > self.max_batch_pixels:
-                break
-            if self.max_batch_samples is not None:
-                est = self._estimate(r)
-                if chosen and work + est > self.max_batch_samples:
+        chosen, pixels, work, waiting, popped = [], 0, 0.0, deque(), []
+        try:
+            while self._queue and len(chosen) < self.max_batch_requests:
+                r = self._queue[0]
+                px = r.width * r.height
+                if chosen and pixels + px > self.max_batch_pixels:
                     break
-                work += est
-            self._queue.popleft()
-            # pin before loading others, so a request's volume is not evicted
-            # by the next request of the same batch
-            if self._make_resident(r.volume):
-                self._resident[r.volume].pins += 1
-                chosen.append(r)
-                pixels += px
-            else:
-                waiting.append(r)
+                if self.max_batch_samples is not None:
+                    est = self._estimate(r)
+                    if chosen and work + est > self.max_batch_samples:
+                        break
+                    work += est
+                popped.append(self._queue.popleft())
+                # pin before loading others, so a request's volume is not
+                # evicted by the next request of the same batch
+                if self._make_resident(r.volume):
+                    self._resident[r.volume].pins += 1
+                    chosen.append(r)
+                    pixels += px
+                else:
+                    waiting.append(r)
+        except BaseException:
+            # a volume could not be loaded: the queue and the pins as before
+            for c in chosen:
+                self._resident[c.volume].pins -= 1
+            self._queue.extendleft(reversed(popped))
+            raise
         self._queue.extendleft(reversed(waiting))
         if not chosen:
             return False
-        for r in chosen:
-            vol = self._registry[r.volume][0]
-            self.backend.add(vol, r.camera, r.ctf, r.otf, r.width, r.height, r.config, r.light, r.tau)
-        k = self.backend.submit()
+        added = 0
+        try:
+            for r in chosen:
+                vol = self._registry[r.volume][0]
+                self.backend.add(vol, r.camera, r.ctf, r.otf, r.width, r.height, r.config, r.light, r.tau)
+                added += 1
+            k = self.backend.submit()
+        except Exception as e:
+            # drop the partial batch and release the pins; a request the
+            # batched path rejects completes with the error and the others
+            # go back to the front of the queue, while a batch that fails to
+            # launch completes every request in it with the error
+            self.backend.clear()
+            for r in chosen:
+                self._resident[r.volume].pins -= 1
+            if added < len(chosen):
+                self._fail(chosen[added], e)
+                self._queue.extendleft(reversed([r for i, r in enumerate(chosen) if i != added]))
+            else:
+                for r in chosen:
+                    self._fail(r, e)
+            return True
         now = time.perf_counter()
         for r in chosen:
             r.launched = now

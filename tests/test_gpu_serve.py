@@ -82,3 +82,38 @@ def test_served_camera_path_equals_single_renders(hip):
             assert r.rgba.tobytes() == img and cnt.sum() > 0
     finally:
         srv.close()
+
+
+def test_rejected_request_does_not_stop_the_server(hip):
+    """A request the batched path cannot take (a LUT of 4096 entries; the
+    batch kernels hold at most 2048 in LDS) comes back with an error and
+    no pixels; the requests around it render as they do alone."""
+    import copy
+
+    from pyvr_amd.hip_renderer.serve import RenderServer, ServeRequest
+
+    vols = scenes.c3_volumes(scale=4)
+    reqs = scenes.c3_requests(5, seed=43, image_scale=4)
+    srv = RenderServer(scalar_format="f16", normal_format="sn8x3", max_batch_requests=8, pipeline=2)
+    try:
+        for k, v in enumerate(vols):
+            srv.register(k, v)
+        sub = []
+        for i, q in enumerate(reqs):
+            otf = q.otf
+            if i == 2:
+                otf = copy.deepcopy(q.otf)
+                otf.lut_size = 4096
+            sub.append(srv.submit(ServeRequest(volume=q.volume, camera=q.camera, ctf=q.ctf, otf=otf,
+                                               width=q.width, height=q.height, config=q.config, light=q.light,
+                                               tau=q.tau)))
+        done = srv.drain()
+        assert sorted(map(id, done)) == sorted(map(id, sub)) and srv.failed == 1
+        assert sub[2].error is not None and "2048" in sub[2].error and sub[2].rgba is None
+        for i, (q, r) in enumerate(zip(reqs, sub)):
+            if i == 2:
+                continue
+            img, _ = single(q, vols[q.volume], "f16", "sn8x3")
+            assert r.error is None and r.rgba.tobytes() == img
+    finally:
+        srv.close()

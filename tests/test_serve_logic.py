@@ -51,6 +51,12 @@ class FakeBackend:
         self.reuse = True
         return k
 
+    def clear(self):
+        # as BatchRenderer.clear: drop the batch being filled
+        assert not self.busy[self.cur]
+        self.slots[self.cur] = []
+        self.reuse = False
+
     def done(self, k):
         return not self.busy[k]
 
@@ -381,3 +387,102 @@ def test_estimate_uses_the_camera_distance_bound():
     # inside the bounding sphere every pixel may hit the box
     assert estimated_samples(128, 128, inside, cfg) == pytest.approx(128 * 128 * min(cfg.max_steps,
                                                                                     4 * 8 / 24 / cfg.step_size))
+
+
+class RejectingBackend(FakeBackend):
+    """Rejects requests 13 pixels wide (as BatchRenderer.add rejects a LUT
+    the batch path cannot hold), fails the launches listed in
+    `fail_launches` (by launch attempt) and uploads of volumes in
+    `fail_uploads`."""
+
+    def __init__(self, pipeline=2, fail_launches=(), fail_uploads=()):
+        super().__init__(pipeline)
+        self.fail_launches, self.fail_uploads, self.attempts = set(fail_launches), set(fail_uploads), 0
+
+    def add(self, vol, camera, ctf, otf, w, h, config, light, tau):
+        if self.reuse:
+            assert not self.busy[self.cur]
+            self.slots[self.cur] = []
+            self.reuse = False
+        if w == 13:
+            raise RuntimeError("hvr_batch_add failed (-1): batched LUTs hold 0 to 2048 entries")
+        super().add(vol, camera, ctf, otf, w, h, config, light, tau)
+
+    def submit(self):
+        self.attempts += 1
+        if self.attempts in self.fail_launches:
+            raise RuntimeError("hvr_batch_run_async failed (-4): capturing the batch graph failed")
+        return super().submit()
+
+    def upload(self, v, device_normals=False):
+        if v.name in self.fail_uploads:
+            raise RuntimeError("hvr_volume_create failed (-2): out of memory")
+        super().upload(v, device_normals)
+
+
+def make_rejecting(**kw):
+    be = RejectingBackend(**kw)
+    reserve = 2 * batch_pool_bytes(256, 1 << 12, volume_bricks((16,) * 3))
+    srv = RenderServer(backend=be, memory_budget=reserve + 3 * B16 + 1, pipeline=2, max_batch_pixels=1 << 12)
+    for k in "ABC":
+        srv.register(k, vol(k, 16))
+    return srv, be
+
+
+def finish_all(be):
+    for k in range(len(be.busy)):
+        be.finish(k)
+
+
+def test_rejected_request_completes_with_error_and_the_rest_render():
+    srv, be = make_rejecting()
+    reqs = [srv.submit(req("A")), srv.submit(req("B", 13, 9)), srv.submit(req("C"))]
+    out = srv.poll()
+    # the rejected request comes back at once with its error; the others
+    # were put back in order and launched as one batch
+    assert out == [] and srv.failed == 1
+    assert [[n for n, _, _ in l] for l in be.launches] == [["A", "C"]]
+    finish_all(be)
+    got = srv.drain()
+    bad = reqs[1]
+    assert bad.error is not None and "2048" in bad.error and bad.rgba is None and bad.done is not None
+    # every request comes back exactly once, the rejected one included
+    assert sorted(map(id, got)) == sorted(map(id, reqs))
+    ok = [r for r in reqs if r is not bad]
+    assert all(r.error is None and r.rgba is not None and r.rgba.size == 16 * 16 * 4 for r in ok)
+    assert srv.pending() == 0 and all(v.pins == 0 for v in srv._resident.values())
+
+
+def test_rejected_request_is_returned_by_poll():
+    srv, be = make_rejecting()
+    bad = srv.submit(req("A", 13, 5))
+    assert srv.poll() == []                 # rejected during this poll's launch
+    assert srv.poll() == [bad] and bad.error and srv.pending() == 0
+    assert all(v.pins == 0 for v in srv._resident.values()) and be.launches == []
+
+
+def test_failed_launch_completes_its_requests_with_the_error():
+    srv, be = make_rejecting(fail_launches={1})
+    first = [srv.submit(req(k)) for k in "AB"]
+    srv.poll()                              # the first launch fails
+    assert all("capturing" in r.error and r.rgba is None for r in first)
+    assert srv.poll() == first and srv.failed == 2
+    again = [srv.submit(req(k)) for k in "AB"]
+    srv.poll()                              # the next launch succeeds
+    finish_all(be)
+    done = srv.drain()
+    assert done == again and all(r.error is None and r.rgba is not None for r in again)
+    assert all(v.pins == 0 for v in srv._resident.values())
+
+
+def test_failed_upload_restores_the_queue_and_the_pins():
+    srv, be = make_rejecting(fail_uploads={"B"})
+    reqs = [srv.submit(req(k)) for k in "ABC"]
+    with pytest.raises(RuntimeError, match="out of memory"):
+        srv.poll()
+    assert list(srv._queue) == reqs and all(v.pins == 0 for v in srv._resident.values())
+    assert be.launches == []
+    be.fail_uploads.clear()
+    srv.poll()
+    finish_all(be)
+    assert srv.drain() == reqs and all(r.error is None for r in reqs)
