@@ -877,6 +877,7 @@ int hvr_volume_upload_scalar_normals(hvr_volume *v, const float *data)
     if (hipMemcpy(d_field, data, bytes, hipMemcpyHostToDevice) != hipSuccess)
         rc = fail(HVR_EDEVICE, "copying the scalar field to the device failed");
     if (rc == HVR_OK) rc = volume_from_field(v, d_field, true);
+    (void)hipStreamSynchronize(c->stream);   // also after a failure: nothing queued may still read the field
     (void)hipFree(d_field);
     return rc;
 }
@@ -1287,6 +1288,7 @@ int hvr_touched_bricks(hvr_ctx *c, unsigned long long *touched, unsigned long lo
     if (rc == HVR_OK && (hipMemcpyAsync(marks.data(), d_marks, nb, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
                          hipStreamSynchronize(c->stream) != hipSuccess))
         rc = fail(HVR_EDEVICE, "hvr_touched_bricks: reading the marks failed");
+    (void)hipStreamSynchronize(c->stream);   // nothing queued may still write the marks
     (void)hipFree(d_marks);
     if (rc != HVR_OK) return rc;
     unsigned long long n = 0;
