@@ -304,7 +304,9 @@ class RenderServer:
         return req
 
     def pending(self) -> int:
-        return len(self._queue) + sum(len(v) for v in self._flight.values())
+        """Submitted requests that poll has not returned yet (queued, in
+        flight, or failed and about to be returned)."""
+        return len(self._queue) + len(self._failed) + sum(len(v) for v in self._flight.values())
 
     def cancel(self, req: ServeRequest) -> bool:
         """Withdraw a submitted request.  A queued request leaves the queue

@@ -486,3 +486,9 @@ def test_failed_upload_restores_the_queue_and_the_pins():
     srv.poll()
     finish_all(be)
     assert srv.drain() == reqs and all(r.error is None for r in reqs)
+
+
+def test_drain_returns_a_request_that_fails_last():
+    srv, be = make_rejecting()
+    bad = srv.submit(req("A", 13, 5))
+    assert srv.drain() == [bad] and bad.error and srv.pending() == 0
