@@ -35,10 +35,14 @@ Default run (what one ``python bench.py`` prints, one JSON line):
   profiles/traffic.json), ``achieved`` that over the launch's HIP-event
   duration, ``frac`` against 8 TB/s.  ``alg`` is the algorithmic figure (8
   texel reads per reference sample at the stored format, plus the images),
-  which counts L1/L2 re-reads and exceeds HBM peak; ``valu`` is the
-  instruction-issue roof that binds the kernel (VALU instructions per sample
-  from the SQ counter pass, over 1024 SIMDs issuing one wave64 instruction
-  per 4 cycles at 2.4 GHz);
+  which counts L1/L2 re-reads and exceeds HBM peak; ``valu`` is the VALU
+  issue of the kernel: VALU instructions per sample from the SQ counter pass
+  against 1024 SIMDs issuing one wave64 instruction per 4 cycles at 2.4 GHz
+  (``frac``; gfx950 issues two VALU instructions in some quad-cycles, so
+  this single-issue figure is no ceiling and can exceed 1), rocprof's
+  VALUBusy (``busy_counter``, from SQ_ACTIVE_INST_VALU), the share of VALU
+  quad-cycles with two instructions issued, the SIMD cycles issuing VALU
+  work (``simd_issue_frac``) and the wave-cycles spent waiting;
 * cpu_baseline: the oracle's FP32 port of the reference shader (its loop
   runs every i < max_steps, as the shader does) on all host cores, on the
   volume's stored values read back from the device, whole frames of the
@@ -188,10 +192,13 @@ def roofline_of(kernel, samples, bpv, out_bytes, ms, key, shaded_fraction=None, 
                         "traffic_over_floor": None if traffic is None else traffic / floor_bytes}
     if "valu_per_sample" in prof:
         va = prof["valu_per_sample"] * samples / t
+        busy, dual = prof.get("valu_busy_counter"), prof.get("valu_dual_issue_frac")
+        simd = busy * (1.0 - dual) if busy is not None and dual is not None else None
         out["valu"] = {"instr_per_sample": prof["valu_per_sample"], "achieved": va / 1e12, "peak": VALU_PEAK / 1e12,
-                       "unit": "T lane-instructions/s", "frac": va / VALU_PEAK,
-                       "busy_counter": prof.get("valu_busy")}
-        out["limiter"] = "valu" if va / VALU_PEAK > (achieved or 0) / HBM_PEAK else "hbm"
+                       "unit": "T lane-instructions/s", "frac": va / VALU_PEAK, "busy_counter": busy,
+                       "dual_issue_frac": dual, "simd_issue_frac": simd, "wave_wait_frac": prof.get("wave_wait_frac")}
+        vf = simd if simd is not None else va / VALU_PEAK
+        out["limiter"] = "valu" if vf > (achieved or 0) / HBM_PEAK else "hbm"
     for k in ("l2_hit", "resident_waves_frac"):
         if k in prof:
             out[k] = prof[k]

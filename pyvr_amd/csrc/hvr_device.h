@@ -904,32 +904,40 @@ __device__ __forceinline__ void march_ray(const FrameConst &fc, const TexDesc &t
             // The in-box run with transparent bricks stepped over: a sample
             // in a transparent brick is counted without loading texels.  The
             // flag is loaded again only when the sample's brick changes (a
-            // brick spans several steps), which saves most flag loads and
-            // their waits.  A footprint outside the box is never used.
+            // brick spans several steps), which saves most flag loads.  Each
+            // iteration first advances to the next sample (its position, box
+            // test and brick: none depends on the composite) and issues that
+            // sample's flag load, then shades the current sample: the flag
+            // load is in flight together with the current sample's texel
+            // loads, so a wave waits for memory once per iteration, not
+            // twice (some lane of a wave changes brick on most iterations).
+            // A footprint outside the box is never used.
             float ax = __fadd_rn(x, bqx), ay = __fadd_rn(y, bqy), az = __fadd_rn(z, bqz);
             bool in = i < nlim && (az <= lz) & (ay <= ly) & (ax <= lx);
             Footprint f = footprint_brick<LB, BUF>(ts, in ? fabsf(az) : 0.f, in ? fabsf(ay) : 0.f,
                                                    in ? fabsf(ax) : 0.f);
             uint32_t cur = f.brick;
-            bool cflag = ld_global(skip + cur) != 0;
-            bool clear = in && cflag;
+            uint32_t flag = ld_global(skip + cur);
+            bool clear = in && flag != 0;
             while (in && A < fc.tau) {
-                if (!clear) shade(f);
                 ++i;
-                if constexpr (COUNT) ns += (fc.count_shaded && clear) ? 0u : 1u;
                 x = __fadd_rn(x, dqx);
                 y = __fadd_rn(y, dqy);
                 z = __fadd_rn(z, dqz);
                 ax = __fadd_rn(x, bqx);
                 ay = __fadd_rn(y, bqy);
                 az = __fadd_rn(z, bqz);
-                in = i < nlim && (az <= lz) & (ay <= ly) & (ax <= lx);
-                f = footprint_brick<LB, BUF>(ts, fabsf(az), fabsf(ay), fabsf(ax));
-                if (in && f.brick != cur) {
-                    cur = f.brick;
-                    cflag = ld_global(skip + cur) != 0;
+                const bool in_next = i < nlim && (az <= lz) & (ay <= ly) & (ax <= lx);
+                const Footprint f_next = footprint_brick<LB, BUF>(ts, fabsf(az), fabsf(ay), fabsf(ax));
+                if (in_next && f_next.brick != cur) {
+                    cur = f_next.brick;
+                    flag = ld_global(skip + cur);
                 }
-                clear = in && cflag;
+                if (!clear) shade(f);
+                if constexpr (COUNT) ns += (fc.count_shaded && clear) ? 0u : 1u;
+                f = f_next;
+                in = in_next;
+                clear = in && flag != 0;
             }
         }
     }
@@ -1306,10 +1314,14 @@ struct TileShape<1> {
 
 // W: waves per block.  SKIP: step over transparent bricks (the flags of
 // k_batch_flags).  GATE: interpolate and light normals only for samples of
-// non-zero alpha.  Every variant gives the same image.
+// non-zero alpha.  Every variant gives the same image.  For texels of up
+// to 8 bytes the launch bounds ask for 8 waves per SIMD (64 VGPRs): the
+// compiler then keeps a few per-ray values in scratch outside the march
+// loop rather than run 7 waves.  16-byte texels would spill inside the
+// loop, so their kernels keep the registers they need.
 template <int SF, int NM, int LB, bool COUNT, int TS, bool BUF, int W = kBatchWaves, bool SKIP = true,
           bool GATE = true>
-__global__ void __launch_bounds__(64 * W)
+__global__ void __launch_bounds__(64 * W, texel_bytes(SF) > 8 ? 1 : 8)
 k_batch(const char *__restrict__ block, int cls, uint32_t *__restrict__ counter)
 {
     extern __shared__ float4 s_lut[];
