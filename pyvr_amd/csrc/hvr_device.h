@@ -682,12 +682,13 @@ enum NormalMode : int {
 // LUT lookup at (rho, 0.5) with u = rho*N - 1/2, clamp to edge
 // (manager.py:163-181; volume.frag.glsl:96).  `lut` holds, per entry i, the
 // pair (entry i, entry i+1 - entry i) (difference 0 after the last), so that
-// clamping u to [0, N-1] gives both edge cases and one fma per channel.  u
-// >= 0 after the clamp (fmaxf also maps a NaN to 0): truncation is floor and
+// clamping u to [0, N-1] gives both edge cases and one fma per channel.  The
+// clamp is one v_med3_f32 (which maps a NaN to min(0, N-1) = 0, as
+// fminf(fmaxf(NaN, 0), N-1) does), so u >= 0: truncation is floor and
 // v_fract_f32 gives u - floor(u) exactly.
 __device__ __forceinline__ float4 lut_sample(const float4 *lut, float fn1, float fn, float rho)
 {
-    const float u = fminf(fmaxf(fmaf(rho, fn, -0.5f), 0.f), fn1);
+    const float u = __builtin_amdgcn_fmed3f(fmaf(rho, fn, -0.5f), 0.f, fn1);
     const float a = __builtin_amdgcn_fractf(u);
     const int i = (int)u;
     const float4 v = lut[2 * i], d = lut[2 * i + 1];
@@ -719,6 +720,19 @@ __device__ __forceinline__ float lambert(const FrameConst &fc, float n0, float n
     const float nn = fmaf(n0, n0, fmaf(n1, n1, n2 * n2));
     const float dt = fmaf(n0, fc.L[0], fmaf(n1, fc.L[1], n2 * fc.L[2])) * __builtin_amdgcn_rsqf(nn);
     return (fc.L_ok && nn >= 1.17549435e-38f) ? fmaxf(dt, 0.f) : 0.f;
+}
+
+// The composite with the light factor lam = ka + kd * diffuse given.
+__device__ __forceinline__ void composite_lam(const FrameConst &fc, const float4 &q, float lam, float &C0,
+                                              float &C1, float &C2, float &A)
+{
+    const float a = 1.f - __builtin_amdgcn_exp2f(-(q.w * fc.alpha_k));
+    const float ta = (1.f - A) * a;
+    const float tl = ta * lam;
+    C0 = fmaf(tl, q.x, C0);
+    C1 = fmaf(tl, q.y, C1);
+    C2 = fmaf(tl, q.z, C2);
+    A = A + ta;
 }
 
 __device__ __forceinline__ void composite_diff(const FrameConst &fc, const float4 &q, float diff, float &C0,
@@ -849,18 +863,18 @@ __device__ __forceinline__ void march_ray(const FrameConst &fc, const TexDesc &t
                 float w[8];
                 fp_weights(f, w);
                 const float rho = interp_scalar<SF>(raw, w) * u8k;
-                float diff = 0.f;
                 if constexpr (GATE) {
                     // Normals only for a non-zero alpha: with alpha 0 the
                     // sample adds (1-A)*0*lambda = 0 for any finite lambda
                     // (the Lambert term is 0 or fmaxf(dot, 0)), so skipping
                     // the normal changes nothing; the texel loads are shared.
                     const float4 q = lut_sample(s_lut, fn1, fn, rho);
+                    float lam = fc.ka;   // fma(kd, 0, ka) = ka exactly (kd finite)
                     if (q.w != 0.f) {
                         const float3 n = interp_normal<SF>(raw, w);
-                        diff = lambert(fc, n.x, n.y, n.z);
+                        lam = fmaf(fc.kd, lambert(fc, n.x, n.y, n.z), fc.ka);
                     }
-                    composite_diff(fc, q, diff, C0, C1, C2, A);
+                    composite_lam(fc, q, lam, C0, C1, C2, A);
                 } else {
                     const float3 n = interp_normal<SF>(raw, w);
                     const float4 q = lut_sample(s_lut, fn1, fn, rho);

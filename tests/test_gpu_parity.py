@@ -289,3 +289,41 @@ def test_touched_bricks(hip):
           f"{clear_on} / {clear_off}", flush=True)
     assert 0 < on <= off <= nb and bb > 0
     assert clear_on == 0 and off <= clear_off <= nb
+
+
+def test_nan_densities_match_oracle(hip):
+    """NaN voxels in an FP32 volume: a sample whose footprint holds one
+    interpolates to NaN and reads LUT entry 0 (the clamp's max(NaN, 0) = 0);
+    its normal is NaN and lights nothing.  With an opacity that is non-zero
+    at density 0 such samples are visible.  Single-frame and batched renders
+    against the FP32-mode oracle; the batch must equal the single render."""
+    from pyvr_amd.datasets import compute_normal_volume
+    from pyvr_amd.hip_renderer import BatchRenderer
+    from pyvr_amd.transferfunctions import OpacityTransferFunction
+
+    base = scenes.c0(size=48, image=96)
+    rng = np.random.default_rng(7)
+    data = base.data.astype(np.float32).copy()
+    data[rng.random(data.shape) < 0.01] = np.nan
+    with np.errstate(invalid="ignore"):
+        normals = compute_normal_volume(data)
+    sc = scenes.Scene(name="c0_nan", data=data, normals=normals, camera=base.camera, light=base.light,
+                      ctf=base.ctf, otf=OpacityTransferFunction.linear(0.05, 0.9), config=base.config,
+                      width=base.width, height=base.height, tau=base.tau)
+    img, cnt = render_hip(sc, counts=True)
+    r32 = oracle_render(sc, mode="f32")
+    npix = sc.width * sc.height
+    st = compare(img, r32["rgba8"])
+    diff = int((cnt.astype(np.int64) != r32["counts"].astype(np.int64)).sum())
+    print(f"[nan] vs FP32 oracle {st}; counts differ {diff}, samples {int(cnt.sum())}", flush=True)
+    assert img[..., 3].max() > 0
+    assert st["n_gt1"] <= max(1, int(1e-3 * npix)), st
+    assert diff <= max(1, int(1e-3 * npix)), diff
+    b = BatchRenderer(scalar_format="f32", normal_format="f32x3")
+    try:
+        i = b.add(sc.volume(), sc.camera, sc.ctf, sc.otf, sc.width, sc.height, sc.config, sc.light, sc.tau)
+        b.run(count=True)
+        got = np.frombuffer(b.result(i), np.uint8).reshape(sc.height, sc.width, 4)
+        assert np.array_equal(got, img) and np.array_equal(b.counts(i), cnt)
+    finally:
+        b.close()
