@@ -797,10 +797,8 @@ __device__ __forceinline__ bool inbox(const FrameConst &fc, float wx, float wy, 
 // samples ns (COUNT).  s_lut: the LUT with its differences in LDS (lut_sample).
 // GATE: interpolate and light the normal only for samples of non-zero
 // alpha (fewer instructions where the march is instruction-bound; the
-// result is identical either way).  UNROLL: the transparent-brick loop
-// unrolled by two (the single-frame kernel is faster with it, the batched
-// one on the 24-camera workload slightly slower; same images).
-template <int SF, int NM, int LB, bool COUNT, bool SKIP, bool BUF, bool GATE = false, bool UNROLL = false>
+// result is identical either way).
+template <int SF, int NM, int LB, bool COUNT, bool SKIP, bool BUF, bool GATE = false>
 __device__ __forceinline__ void march_ray(const FrameConst &fc, const TexDesc &ts, const TexDesc &tn,
                                           const float4 *__restrict__ lut_g, int lut_n, const float4 *s_lut,
                                           const RayRec &r, const uint8_t *__restrict__ skip,
@@ -930,63 +928,35 @@ __device__ __forceinline__ void march_ray(const FrameConst &fc, const TexDesc &t
             // A footprint outside the box is never used.
             float ax = __fadd_rn(x, bqx), ay = __fadd_rn(y, bqy), az = __fadd_rn(z, bqz);
             bool in = i < nlim && (az <= lz) & (ay <= ly) & (ax <= lx);
-            if constexpr (UNROLL) {
-                // Unrolled by two, two footprints trading roles (no register
-                // copies between iterations).  The flag held is always that
-                // of the current sample's brick.
-                Footprint fa = footprint_brick<LB, BUF>(ts, in ? fabsf(az) : 0.f, in ? fabsf(ay) : 0.f,
-                                                        in ? fabsf(ax) : 0.f),
-                          fb;
-                uint32_t flag = ld_global(skip + fa.brick);
-                bool clear = in && flag != 0;
-                // one sample: f the current sample's footprint, g receives the next one's
-                auto step = [&](const Footprint &f, Footprint &g) {
-                    ++i;
-                    x = __fadd_rn(x, dqx);
-                    y = __fadd_rn(y, dqy);
-                    z = __fadd_rn(z, dqz);
-                    ax = __fadd_rn(x, bqx);
-                    ay = __fadd_rn(y, bqy);
-                    az = __fadd_rn(z, bqz);
-                    const bool in_next = i < nlim && (az <= lz) & (ay <= ly) & (ax <= lx);
-                    g = footprint_brick<LB, BUF>(ts, fabsf(az), fabsf(ay), fabsf(ax));
-                    if (in_next && g.brick != f.brick) flag = ld_global(skip + g.brick);
-                    if (!clear) shade(f);
-                    if constexpr (COUNT) ns += (fc.count_shaded && clear) ? 0u : 1u;
-                    in = in_next;
-                    clear = in && flag != 0;
-                };
-                while (in && A < fc.tau) {
-                    step(fa, fb);
-                    if (!(in && A < fc.tau)) break;
-                    step(fb, fa);
-                }
-            } else {
-                Footprint f = footprint_brick<LB, BUF>(ts, in ? fabsf(az) : 0.f, in ? fabsf(ay) : 0.f,
-                                                       in ? fabsf(ax) : 0.f);
-                uint32_t cur = f.brick;
-                uint32_t flag = ld_global(skip + cur);
-                bool clear = in && flag != 0;
-                while (in && A < fc.tau) {
-                    ++i;
-                    x = __fadd_rn(x, dqx);
-                    y = __fadd_rn(y, dqy);
-                    z = __fadd_rn(z, dqz);
-                    ax = __fadd_rn(x, bqx);
-                    ay = __fadd_rn(y, bqy);
-                    az = __fadd_rn(z, bqz);
-                    const bool in_next = i < nlim && (az <= lz) & (ay <= ly) & (ax <= lx);
-                    const Footprint f_next = footprint_brick<LB, BUF>(ts, fabsf(az), fabsf(ay), fabsf(ax));
-                    if (in_next && f_next.brick != cur) {
-                        cur = f_next.brick;
-                        flag = ld_global(skip + cur);
-                    }
-                    if (!clear) shade(f);
-                    if constexpr (COUNT) ns += (fc.count_shaded && clear) ? 0u : 1u;
-                    f = f_next;
-                    in = in_next;
-                    clear = in && flag != 0;
-                }
+            // Unrolled by two, two footprints trading roles (no register
+            // copies between iterations).  The flag held is always that
+            // of the current sample's brick.
+            Footprint fa = footprint_brick<LB, BUF>(ts, in ? fabsf(az) : 0.f, in ? fabsf(ay) : 0.f,
+                                                    in ? fabsf(ax) : 0.f),
+                      fb;
+            uint32_t flag = ld_global(skip + fa.brick);
+            bool clear = in && flag != 0;
+            // one sample: f the current sample's footprint, g receives the next one's
+            auto step = [&](const Footprint &f, Footprint &g) {
+                ++i;
+                x = __fadd_rn(x, dqx);
+                y = __fadd_rn(y, dqy);
+                z = __fadd_rn(z, dqz);
+                ax = __fadd_rn(x, bqx);
+                ay = __fadd_rn(y, bqy);
+                az = __fadd_rn(z, bqz);
+                const bool in_next = i < nlim && (az <= lz) & (ay <= ly) & (ax <= lx);
+                g = footprint_brick<LB, BUF>(ts, fabsf(az), fabsf(ay), fabsf(ax));
+                if (in_next && g.brick != f.brick) flag = ld_global(skip + g.brick);
+                if (!clear) shade(f);
+                if constexpr (COUNT) ns += (fc.count_shaded && clear) ? 0u : 1u;
+                in = in_next;
+                clear = in && flag != 0;
+            };
+            while (in && A < fc.tau) {
+                step(fa, fb);
+                if (!(in && A < fc.tau)) break;
+                step(fb, fa);
             }
         }
     }
@@ -1041,7 +1011,7 @@ k_march(const FrameConst fc, const TexDesc ts, const TexDesc tn, const float4 *_
 
     float C0 = 0.f, C1 = 0.f, C2 = 0.f, A = 0.f;
     uint32_t ns = 0;
-    march_ray<SF, NM, LB, COUNT, SKIP, BUF, false, true>(fc, ts, tn, lut_g, lut_n, s_lut, r, skip, C0, C1, C2, A, ns);
+    march_ray<SF, NM, LB, COUNT, SKIP, BUF>(fc, ts, tn, lut_g, lut_n, s_lut, r, skip, C0, C1, C2, A, ns);
     out[pix] = blend_store(fc, C0, C1, C2, A);
     if constexpr (COUNT) counts[pix] = ns;
 }
