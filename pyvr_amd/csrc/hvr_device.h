@@ -1351,7 +1351,10 @@ k_batch(const char *__restrict__ block, int cls, uint32_t *__restrict__ counter)
     if (nreq <= 0) return;
     const ReqDesc *__restrict__ reqs = (const ReqDesc *)(block + hdr->cls[cls].desc_offset);
     const int total = reqs[nreq - 1].tile_end;
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // the wave's index in the block as a scalar, so that no per-thread value
+    // but the lane index is needed after a ray's march (the pixel is formed
+    // again there from scalars and the lane index)
+    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x) >> 6, lane = threadIdx.x & 63;
     int cur = -1;
     for (;;) {
         if (threadIdx.x == 0) {
@@ -1389,7 +1392,8 @@ k_batch(const char *__restrict__ block, int cls, uint32_t *__restrict__ counter)
             uint32_t ns = 0;
             march_ray<SF, NM, LB, COUNT, SKIP, BUF, GATE>(R.fc, R.ts, R.ts, R.lut, R.lut_n, s_lut, r,
                                                            R.flags, C0, C1, C2, A, ns);
-            const int pix = py * R.fc.W + px;
+            const int ln = (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));   // = lane
+            const int pix = (ty * BY + (wv >> 1) * WY + ln / WX) * R.fc.W + tx * BX + (wv & 1) * WX + ln % WX;
             st_global(R.out + pix, blend_store(R.fc, C0, C1, C2, A));
             if constexpr (COUNT) st_global(R.counts + pix, ns);
         }
